@@ -1,0 +1,179 @@
+/*
+ * lbm.h -- D3Q19 lattice-Boltzmann wind tunnel for AMD Instinct MI355X
+ * (gfx950), C ABI.
+ *
+ * Drop-in for the solver API of MarcosAsh/Lattice_Fluid_Dynamics
+ * (simulation/lib/lbm.h:6-129): the same function names, argument order and
+ * meaning, the same public LBMGrid field names for the flags callers poke
+ * (tau, useRegularized, useMRT, useSmagorinsky, smagorinskyCs, periodicYZ),
+ * the same D3Q19 direction order and the same world box (x in [-4,4],
+ * y,z in [-2,2]).  The OpenGL buffer objects become HIP device pointers.
+ *
+ * Error behaviour follows the reference: LBM_Create returns NULL (and prints
+ * a message) when allocation fails; LBM_InitializeFlow returns 0 on failure;
+ * other calls return void.  LBM_GetLastError() additionally reports the last
+ * device error text.  One HIP stream per grid orders all work; every call
+ * that returns host-visible data synchronises that stream.  A grid is not
+ * thread-safe; separate grids may be driven from separate threads.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Population storage formats (LBMGrid.populationStorage). */
+#define LBM_STORAGE_FP32 0  /* 76 B/cell, reference arithmetic */
+#define LBM_STORAGE_FP16 1  /* 38 B/cell: f_i - w_i in FP16 (x 2^15), FP32 arithmetic */
+#define LBM_STORAGE_BF16 2  /* 38 B/cell: f_i - w_i in BF16, FP32 arithmetic */
+#define LBM_STORAGE_FP64 3  /* 152 B/cell: FP64 storage and arithmetic (oracle runs) */
+
+/* Particle record of simulation/lib/particle_system.h:12-17 (32 bytes). */
+typedef struct {
+    float x, y, z;    /* position (world units) */
+    float padding1;   /* |velocity| after the last advection */
+    float vx, vy, vz; /* velocity (world units per time unit) */
+    float life;
+} Particle;
+
+/* Replaces simulation/lib/lbm.h:6-56.  GL handles are gone; the velocity and
+ * solid buffers are device pointers with the reference layouts. */
+typedef struct {
+    int sizeX, sizeY, sizeZ;
+    int totalCells;
+
+    float tau;
+
+    float *velocityBuffer; /* device, float4 (ux, uy, uz, rho) per cell, x + NX*(y + NY*z) */
+    int *solidBuffer;      /* device, int per cell: 0 fluid, 1 body, 2 ground */
+
+    int useRegularized;  /* 0 = BGK, 1 = regularized */
+    int useMRT;          /* 0 = off, 1 = MRT collision operator */
+    int useSmagorinsky;  /* 0 = off, 1 = Smagorinsky SGS */
+    float smagorinskyCs; /* Smagorinsky constant */
+    int periodicYZ;      /* 0 = clamp, 1 = periodic y/z */
+
+    /* Z-slab fields of the reference (GL SSBO size limit); always one slab. */
+    int numChunks;
+    int slabZ;
+    int haloZ;
+
+    /* Additions (no reference counterpart). */
+    int forceFixedPoint;   /* 1: accumulate the force as the reference does, round(F*1e4)
+                              per boundary cell in integers; 0 (default): FP64 sum */
+    int populationStorage; /* LBM_STORAGE_*; takes effect at LBM_InitializeFlow */
+    long long stepCount;   /* LBM_Step calls since LBM_InitializeFlow */
+    void *impl;            /* opaque */
+} LBMGrid;
+
+/* simulation/lib/lbm.h:59  tau = 0.5 + 3*viscosity clamped to [0.52, 2]. */
+LBMGrid *LBM_Create(int sizeX, int sizeY, int sizeZ, float viscosity);
+
+/* simulation/lib/lbm.h:62  NULL-safe. */
+void LBM_Free(LBMGrid *grid);
+
+/* simulation/lib/lbm.h:65-71  world AABB -> solid=1, q=0.5 on every fluid->solid link. */
+void LBM_SetSolidAABB(LBMGrid *grid, float minX, float minY, float minZ, float maxX, float maxY,
+                      float maxZ);
+
+/* simulation/lib/lbm.h:74  f = f_new = feq(1, u), velocity = (u, 1). 1 on success, 0 on failure. */
+int LBM_InitializeFlow(LBMGrid *grid, float ux, float uy, float uz);
+
+/* simulation/lib/lbm.h:77  one collide + stream step with the given inlet velocity. */
+void LBM_Step(LBMGrid *grid, float inletVelX, float inletVelY, float inletVelZ);
+
+/* simulation/lib/lbm.h:80  (GLuint -> device pointer) float4 per cell: the
+ * pre-collision moments of the last step.  Work is queued on the grid's
+ * stream: call LBM_Synchronize before reading it from another stream. */
+float *LBM_GetVelocityBuffer(LBMGrid *grid);
+
+/* simulation/lib/lbm.h:83-86  momentum-exchange force on the body. */
+void LBM_ComputeDragForce(LBMGrid *grid, float *forceX, float *forceY, float *forceZ);
+
+/* simulation/lib/lbm.h:90-96  force with the equilibrium ("pressure") part;
+ * pressure outputs may be NULL.  Outputs are zeroed first. */
+void LBM_ComputeDragForceDecomposed(LBMGrid *grid, float *forceX, float *forceY, float *forceZ,
+                                    float *pressureX, float *pressureY, float *pressureZ);
+
+/* simulation/lib/lbm.h:99-101  |Fx| / (0.5 U^2 A). */
+float LBM_ComputeDragCoefficient(LBMGrid *grid, float inletVelocity, float refArea);
+
+/* simulation/lib/lbm.h:104-106  |Fy| / (0.5 U^2 A). */
+float LBM_ComputeLiftCoefficient(LBMGrid *grid, float inletVelocity, float refArea);
+
+/* simulation/lib/lbm.h:109-110  sphere, analytic Bouzidi q. */
+void LBM_SetSolidSphere(LBMGrid *grid, float cx, float cy, float cz, float radius);
+
+/* simulation/lib/lbm.h:114  ground cells below worldZ become solid=2. */
+void LBM_AddGroundPlane(LBMGrid *grid, float worldZ);
+
+/* simulation/lib/lbm.h:118  columns holding a solid==1 cell along axis (0=x, 1=y, 2=z). */
+float LBM_ComputeProjectedArea(LBMGrid *grid, int axis);
+
+/* simulation/lib/lbm.h:121-129  triangles: 12 floats each (v0 xyz pad, v1 xyz pad,
+ * v2 xyz pad; simulation/lib/model_bounds.h:14-18); solid by +X ray parity,
+ * q by ray casting along the links. */
+void LBM_SetSolidMesh(LBMGrid *grid, float *triangles, int numTriangles, float minX, float minY,
+                      float minZ, float maxX, float maxY, float maxZ);
+
+/* ---- additions replacing direct GL access by callers --------------------- */
+
+/* n LBM_Step calls with the same inlet velocity.  Observable state afterwards
+ * is identical to the n single calls (the velocity buffer, which every step
+ * overwrites, is written on the last one only). */
+void LBM_StepN(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inletVelZ);
+
+/* Replaces glGetBufferSubData on velocityBuffer (test_lbm.c:503-505,
+ * vti_export.c:30-34): 4 floats per cell.  Returns 1 on success. */
+int LBM_CopyVelocityToHost(LBMGrid *grid, float *out);
+
+/* Replaces glGetBufferSubData on solidBuffer (test_lbm.c:258-260). */
+int LBM_CopySolidToHost(LBMGrid *grid, int *out);
+
+/* The reference's two population buffers in their layout f[i + 19*cell]:
+ * post-stream f and post-collision f_new of the last step (either may be NULL). */
+int LBM_CopyPopulationsToHost(LBMGrid *grid, float *f, float *fNew);
+/* The same in double precision (exact for every storage format). */
+int LBM_CopyPopulationsToHostF64(LBMGrid *grid, double *f, double *fNew);
+
+/* Sum of all post-stream populations (total mass) accumulated in FP64. */
+double LBM_ComputeTotalMass(LBMGrid *grid);
+
+/* FP64 force sums: out[0..2] total force, out[3..5] equilibrium part. */
+void LBM_ComputeDragForceF64(LBMGrid *grid, double out[6]);
+
+/* Particles (the reference's particle dispatch is inline GL in main.c:1012-1101).
+ * `particles` is device memory (LBM_DeviceAlloc).  Seeding puts n particles on
+ * the inlet plane x=-4 with y,z uniform in (-2,2) from `seed`.  Advection is
+ * midpoint RK2 through the trilinear velocity sampler of
+ * particle_lbm.comp:46-90, displacement = dt * velScale * u_lattice; particles
+ * leaving the box respawn on the inlet plane from (seed, step). */
+void LBM_SeedParticles(LBMGrid *grid, Particle *particles, int n, unsigned int seed);
+void LBM_AdvectParticles(LBMGrid *grid, Particle *particles, int n, float dt, float velScale,
+                         unsigned int seed, unsigned int step);
+
+void *LBM_DeviceAlloc(unsigned long long bytes);
+void LBM_DeviceFree(void *p);
+int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes);
+int LBM_CopyToHost(void *dst, const void *src, unsigned long long bytes);
+
+/* Waits for all queued work of the grid. */
+void LBM_Synchronize(LBMGrid *grid);
+
+/* Runs n steps and reports HIP-event times on the grid's stream: total_ms for
+ * the whole region, main_ms and boundary_ms summed over the fused step kernel
+ * and the boundary prepass launches (NULL to skip per-launch events). */
+int LBM_ProfileSteps(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inletVelZ,
+                     float *total_ms, float *main_ms, float *boundary_ms);
+
+/* Number of cells in the compacted boundary list used by the prepass. */
+int LBM_GetBoundaryCellCount(LBMGrid *grid);
+
+/* Text of the last device or argument error ("" if none). */
+const char *LBM_GetLastError(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_H */

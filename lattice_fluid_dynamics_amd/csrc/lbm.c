@@ -1,0 +1,841 @@
+/*
+ * Host side of the MI355X D3Q19 solver: the C API of include/lbm.h.
+ *
+ * This file plays the role of the reference's simulation/src/lbm.c: grid
+ * creation, solid/Bouzidi setup on the host, equilibrium initialisation,
+ * step dispatch and the drag/lift post-processing.  All device work goes
+ * through the thin C ABI of lbm_device.h (HIP kernels in lbm_kernels.hip).
+ *
+ * Host geometry state: solid[cell] (0 fluid, 1 body, 2 ground) and a sparse
+ * q table (qRow[cell] -> 19 floats, -1 = no link) replacing the reference's
+ * dense 19-float-per-cell q buffer.  Before a step the host compacts the
+ * boundary cells (cells with links, plus the Y/Z faces when the Y/Z
+ * boundaries clamp, plus body cells on the x faces) into the list that the
+ * device prepass evaluates the reference stream rule on.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/lbm.h"
+#include "lbm_device.h"
+
+/* D3Q19 velocity set, reference order (simulation/src/lbm.c:9-35). */
+static const int kEx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int kEy[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int kEz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+
+typedef struct {
+    LbmDev *dev;
+    int32_t *solid;
+    int32_t *qRow;
+    float *qRows;
+    int nRows, capRows;
+    int geomDirty;
+    int builtPeriodic;
+    int nBoundary;
+    int flowReady;
+} Impl;
+
+static char g_lastError[512];
+
+static void note_error(const char *msg) {
+    snprintf(g_lastError, sizeof(g_lastError), "%s", msg);
+    printf("ERROR: %s\n", msg);
+}
+
+const char *LBM_GetLastError(void) { return g_lastError; }
+
+static Impl *impl_of(LBMGrid *g) { return (Impl *)g->impl; }
+
+static long cell_index(const LBMGrid *g, int x, int y, int z) {
+    return x + (long)g->sizeX * (y + (long)g->sizeY * z);
+}
+
+/* Host equilibrium in float, the expression of simulation/src/lbm.c:37-44:
+ * w rho (1 + eu/cs2 + eu^2/(2 cs2^2) - u2/(2 cs2)). */
+static float host_weight(int i) { return i == 0 ? 1.0f / 3.0f : (i < 7 ? 1.0f / 18.0f : 1.0f / 36.0f); }
+
+static float host_feq(int i, float rho, float ux, float uy, float uz) {
+    const float cs2 = 1.0f / 3.0f;
+    float eu = kEx[i] * ux + kEy[i] * uy + kEz[i] * uz;
+    float uu = ux * ux + uy * uy + uz * uz;
+    return host_weight(i) * rho * (1.0f + eu / cs2 + (eu * eu) / (2.0f * cs2 * cs2) - uu / (2.0f * cs2));
+}
+
+/* ------------------------------------------------------------ q table */
+
+static void q_reset(Impl *im, long ncell) {
+    for (long c = 0; c < ncell; c++) im->qRow[c] = -1;
+    im->nRows = 0;
+}
+
+static float *q_row_for(Impl *im, long cell, int create) {
+    int r = im->qRow[cell];
+    if (r >= 0) return im->qRows + 19L * r;
+    if (!create) return NULL;
+    if (im->nRows == im->capRows) {
+        int cap = im->capRows ? 2 * im->capRows : 4096;
+        float *p = (float *)realloc(im->qRows, sizeof(float) * 19L * cap);
+        if (!p) return NULL;
+        im->qRows = p;
+        im->capRows = cap;
+    }
+    r = im->nRows++;
+    im->qRow[cell] = r;
+    float *row = im->qRows + 19L * r;
+    for (int i = 0; i < 19; i++) row[i] = -1.0f;
+    return row;
+}
+
+static float q_get(Impl *im, long cell, int i) {
+    int r = im->qRow[cell];
+    return r >= 0 ? im->qRows[19L * r + i] : -1.0f;
+}
+
+static void q_set(Impl *im, long cell, int i, float q) {
+    float *row = q_row_for(im, cell, 1);
+    if (row) row[i] = q;
+}
+
+/* ------------------------------------------------------------ create / free */
+
+LBMGrid *LBM_Create(int sizeX, int sizeY, int sizeZ, float viscosity) {
+    if (sizeX < 2 || sizeY < 1 || sizeZ < 1) {
+        note_error("LBM_Create: grid dimensions must be at least 2x1x1");
+        return NULL;
+    }
+    long long cells = (long long)sizeX * sizeY * sizeZ;
+    if (cells > 0x7fffff00LL) {
+        note_error("LBM_Create: grid exceeds 2^31 cells");
+        return NULL;
+    }
+    LBMGrid *g = (LBMGrid *)calloc(1, sizeof(LBMGrid));
+    Impl *im = (Impl *)calloc(1, sizeof(Impl));
+    if (!g || !im) {
+        free(g);
+        free(im);
+        note_error("LBM_Create: host allocation failed");
+        return NULL;
+    }
+    g->impl = im;
+    g->sizeX = sizeX;
+    g->sizeY = sizeY;
+    g->sizeZ = sizeZ;
+    g->totalCells = (int)cells;
+    /* simulation/src/lbm.c:56-60 */
+    g->tau = 0.5f + 3.0f * viscosity;
+    if (g->tau < 0.52f) g->tau = 0.52f;
+    if (g->tau > 2.0f) g->tau = 2.0f;
+    g->useRegularized = 0;
+    g->useMRT = 0;
+    g->useSmagorinsky = 0;
+    g->smagorinskyCs = 0.1f;
+    g->periodicYZ = 0;
+    g->numChunks = 1;
+    g->slabZ = sizeZ;
+    g->haloZ = 1;
+    g->forceFixedPoint = 1;
+    g->populationStorage = LBM_STORAGE_FP32;
+    printf("LBM Grid: %dx%dx%d (%d cells), tau=%.3f\n", sizeX, sizeY, sizeZ, g->totalCells, g->tau);
+
+    im->solid = (int32_t *)calloc((size_t)cells, sizeof(int32_t));
+    im->qRow = (int32_t *)malloc((size_t)cells * sizeof(int32_t));
+    if (!im->solid || !im->qRow) {
+        note_error("LBM_Create: host allocation failed");
+        LBM_Free(g);
+        return NULL;
+    }
+    q_reset(im, (long)cells);
+    im->builtPeriodic = -1;
+    im->geomDirty = 1;
+
+    char err[256] = {0};
+    im->dev = lbmdev_create(sizeX, sizeY, sizeZ, g->populationStorage, err, sizeof(err));
+    if (!im->dev) {
+        char msg[400];
+        snprintf(msg, sizeof(msg), "LBM_Create: %s", err);
+        note_error(msg);
+        LBM_Free(g);
+        return NULL;
+    }
+    g->velocityBuffer = lbmdev_velocity_ptr(im->dev);
+    g->solidBuffer = lbmdev_solid_ptr(im->dev);
+    printf("GPU memory: populations %.1f MB (in place), velocity %.1f MB\n",
+           19.0 * 4.0 * (double)cells / 1048576.0, 16.0 * (double)cells / 1048576.0);
+    return g;
+}
+
+void LBM_Free(LBMGrid *grid) {
+    if (!grid) return;
+    Impl *im = impl_of(grid);
+    if (im) {
+        lbmdev_destroy(im->dev);
+        free(im->solid);
+        free(im->qRow);
+        free(im->qRows);
+        free(im);
+    }
+    free(grid);
+}
+
+/* ------------------------------------------------------------ geometry */
+
+/* World box x in [-4,4], y,z in [-2,2] mapped onto the lattice
+ * (simulation/src/lbm.c:352-354). */
+typedef struct {
+    float sx, sy, sz;
+} Scale;
+
+static Scale world_scale(const LBMGrid *g) {
+    Scale s = {g->sizeX / 8.0f, g->sizeY / 4.0f, g->sizeZ / 4.0f};
+    return s;
+}
+
+/* Visits every in-range link from a cell with solid != skipValue toward a
+ * neighbour that satisfies `isTarget`, in cell order then direction order. */
+typedef float (*LinkQ)(void *ctx, int x, int y, int z, int i);
+
+static void mark_links(LBMGrid *g, int (*isSource)(int32_t), int (*isTarget)(int32_t), LinkQ fq,
+                       void *ctx) {
+    Impl *im = impl_of(g);
+    for (int z = 0; z < g->sizeZ; z++)
+        for (int y = 0; y < g->sizeY; y++)
+            for (int x = 0; x < g->sizeX; x++) {
+                long c = cell_index(g, x, y, z);
+                if (!isSource(im->solid[c])) continue;
+                for (int i = 1; i < 19; i++) {
+                    int X = x + kEx[i], Y = y + kEy[i], Z = z + kEz[i];
+                    if (X < 0 || X >= g->sizeX || Y < 0 || Y >= g->sizeY || Z < 0 || Z >= g->sizeZ) continue;
+                    if (!isTarget(im->solid[cell_index(g, X, Y, Z)])) continue;
+                    q_set(im, c, i, fq(ctx, x, y, z, i));
+                }
+            }
+}
+
+static int not_body(int32_t s) { return s != 1; }
+static int is_body(int32_t s) { return s == 1; }
+static int not_fluid(int32_t s) { return s != 0; }
+static float q_half(void *ctx, int x, int y, int z, int i) {
+    (void)ctx, (void)x, (void)y, (void)z, (void)i;
+    return 0.5f;
+}
+
+static void geometry_changed(LBMGrid *g) {
+    impl_of(g)->geomDirty = 1;
+}
+
+/* simulation/src/lbm.c:346-435 */
+void LBM_SetSolidAABB(LBMGrid *g, float minX, float minY, float minZ, float maxX, float maxY, float maxZ) {
+    if (!g) return;
+    Impl *im = impl_of(g);
+    Scale s = world_scale(g);
+    int lo[3] = {(int)((minX + 4.0f) * s.sx), (int)((minY + 2.0f) * s.sy), (int)((minZ + 2.0f) * s.sz)};
+    int hi[3] = {(int)((maxX + 4.0f) * s.sx), (int)((maxY + 2.0f) * s.sy), (int)((maxZ + 2.0f) * s.sz)};
+    int n[3] = {g->sizeX, g->sizeY, g->sizeZ};
+    for (int a = 0; a < 3; a++) {
+        if (lo[a] < 0) lo[a] = 0;
+        if (hi[a] >= n[a]) hi[a] = n[a] - 1;
+    }
+    printf("LBM solid AABB: grid [%d-%d, %d-%d, %d-%d]\n", lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+    memset(im->solid, 0, sizeof(int32_t) * (size_t)g->totalCells);
+    for (int z = lo[2]; z <= hi[2]; z++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+            for (int x = lo[0]; x <= hi[0]; x++) im->solid[cell_index(g, x, y, z)] = 1;
+    q_reset(im, g->totalCells);
+    mark_links(g, not_body, is_body, q_half, NULL);
+    geometry_changed(g);
+}
+
+typedef struct {
+    Scale s;
+    float cx, cy, cz, r;
+} SphereCtx;
+
+/* Cell centre in world coordinates (simulation/src/lbm.c:446-448). */
+static void cell_centre(Scale s, int x, int y, int z, float *wx, float *wy, float *wz) {
+    *wx = (x + 0.5f) / s.sx - 4.0f;
+    *wy = (y + 0.5f) / s.sy - 2.0f;
+    *wz = (z + 0.5f) / s.sz - 2.0f;
+}
+
+static float clamp_q(float q) {
+    if (q < 0.01f) q = 0.01f;
+    if (q > 0.99f) q = 0.99f;
+    return q;
+}
+
+/* Nearest root of |o + t d|^2 = r^2 with t in (0,1], else 0.5
+ * (simulation/src/lbm.c:492-515). */
+static float sphere_q(void *vctx, int x, int y, int z, int i) {
+    SphereCtx *c = (SphereCtx *)vctx;
+    float wx, wy, wz;
+    cell_centre(c->s, x, y, z, &wx, &wy, &wz);
+    float dx = (float)kEx[i] / c->s.sx, dy = (float)kEy[i] / c->s.sy, dz = (float)kEz[i] / c->s.sz;
+    float ox = wx - c->cx, oy = wy - c->cy, oz = wz - c->cz;
+    float a = dx * dx + dy * dy + dz * dz;
+    float b = 2.0f * (ox * dx + oy * dy + oz * dz);
+    float cc = ox * ox + oy * oy + oz * oz - c->r * c->r;
+    float disc = b * b - 4.0f * a * cc;
+    float q = 0.5f;
+    if (disc >= 0.0f) {
+        float t = (-b - sqrtf(disc)) / (2.0f * a);
+        if (t > 0.0f && t <= 1.0f) q = t;
+    }
+    return clamp_q(q);
+}
+
+/* simulation/src/lbm.c:437-535 */
+void LBM_SetSolidSphere(LBMGrid *g, float cx, float cy, float cz, float radius) {
+    if (!g) return;
+    Impl *im = impl_of(g);
+    SphereCtx ctx = {world_scale(g), cx, cy, cz, radius};
+    int count = 0;
+    for (int z = 0; z < g->sizeZ; z++)
+        for (int y = 0; y < g->sizeY; y++)
+            for (int x = 0; x < g->sizeX; x++) {
+                float wx, wy, wz;
+                cell_centre(ctx.s, x, y, z, &wx, &wy, &wz);
+                float dx = wx - cx, dy = wy - cy, dz = wz - cz;
+                int in = dx * dx + dy * dy + dz * dz <= radius * radius;
+                im->solid[cell_index(g, x, y, z)] = in;
+                count += in;
+            }
+    printf("LBM solid sphere: %d cells (R=%.2f at (%.2f,%.2f,%.2f))\n", count, radius, cx, cy, cz);
+    q_reset(im, g->totalCells);
+    mark_links(g, not_body, is_body, sphere_q, &ctx);
+    geometry_changed(g);
+}
+
+/* simulation/src/lbm.c:537-614 */
+void LBM_AddGroundPlane(LBMGrid *g, float worldZ) {
+    if (!g) return;
+    Impl *im = impl_of(g);
+    int gz = (int)((worldZ + 2.0f) * (g->sizeZ / 4.0f));
+    if (gz < 0) gz = 0;
+    if (gz >= g->sizeZ) gz = g->sizeZ - 1;
+    int groundCells = 0, links = 0;
+    for (int z = 0; z <= gz; z++)
+        for (int y = 0; y < g->sizeY; y++)
+            for (int x = 0; x < g->sizeX; x++) {
+                long c = cell_index(g, x, y, z);
+                if (im->solid[c] == 0) {
+                    im->solid[c] = 2;
+                    groundCells++;
+                }
+            }
+    const int za = gz + 1;
+    if (za < g->sizeZ) {
+        for (int y = 0; y < g->sizeY; y++)
+            for (int x = 0; x < g->sizeX; x++) {
+                long c = cell_index(g, x, y, za);
+                if (im->solid[c] != 0) continue;
+                for (int i = 1; i < 19; i++) {
+                    if (kEz[i] != -1) continue;
+                    int X = x + kEx[i], Y = y + kEy[i];
+                    if (X < 0 || X >= g->sizeX || Y < 0 || Y >= g->sizeY) continue;
+                    if (im->solid[cell_index(g, X, Y, gz)] != 0 && q_get(im, c, i) < 0.0f) {
+                        q_set(im, c, i, 0.5f);
+                        links++;
+                    }
+                }
+            }
+    }
+    printf("Ground plane: z_lattice=%d, %d cells, %d boundary links\n", gz, groundCells, links);
+    geometry_changed(g);
+}
+
+/* simulation/src/lbm.c:616-668 */
+float LBM_ComputeProjectedArea(LBMGrid *g, int axis) {
+    if (!g) return 0.0f;
+    Impl *im = impl_of(g);
+    int n[3] = {g->sizeX, g->sizeY, g->sizeZ};
+    int a = axis == 0 ? 0 : (axis == 1 ? 1 : 2);
+    int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    int count = 0;
+    for (int j = 0; j < n[c]; j++)
+        for (int k = 0; k < n[b]; k++) {
+            for (int t = 0; t < n[a]; t++) {
+                int p[3];
+                p[a] = t;
+                p[b] = k;
+                p[c] = j;
+                if (im->solid[cell_index(g, p[0], p[1], p[2])] == 1) {
+                    count++;
+                    break;
+                }
+            }
+        }
+    return (float)count;
+}
+
+/* ---- triangle mesh ---------------------------------------------------------
+ * Solid: odd number of +X ray crossings from the cell centre, over triangles
+ * whose (y, z) range contains the centre and whose max x is not behind it
+ * (simulation/src/lbm.c:1006-1097).  q: nearest intersection t in (0,1] along
+ * each fluid->solid link, over the SAME candidate set (the reference reuses
+ * the +X cull for every link direction, simulation/src/lbm.c:1150-1157),
+ * fallback 0.5, clamped to [0.01, 0.99].
+ * A uniform (y,z) bin grid narrows the candidates; the exact cull test is
+ * still applied per triangle, so the candidate set (and every result) is the
+ * same as testing all triangles. */
+typedef struct {
+    const float *tri;
+    int ntri;
+    float *bounds; /* per triangle: minY maxY minZ maxZ maxX */
+    int by, bz;
+    float y0, z0, iy, iz;
+    int *binStart, *binList;
+} MeshIndex;
+
+static int bin_of(float v, float o, float inv, int n) {
+    int b = (int)floorf((v - o) * inv);
+    return b < 0 ? 0 : (b >= n ? n - 1 : b);
+}
+
+static int mesh_index_build(MeshIndex *m, const float *tri, int ntri) {
+    memset(m, 0, sizeof(*m));
+    m->tri = tri;
+    m->ntri = ntri;
+    m->bounds = (float *)malloc(sizeof(float) * 5 * (size_t)(ntri > 0 ? ntri : 1));
+    if (!m->bounds) return 0;
+    float ymin = INFINITY, ymax = -INFINITY, zmin = INFINITY, zmax = -INFINITY;
+    for (int t = 0; t < ntri; t++) {
+        const float *v = tri + 12L * t;
+        float *b = m->bounds + 5L * t;
+        b[0] = fminf(v[1], fminf(v[5], v[9]));
+        b[1] = fmaxf(v[1], fmaxf(v[5], v[9]));
+        b[2] = fminf(v[2], fminf(v[6], v[10]));
+        b[3] = fmaxf(v[2], fmaxf(v[6], v[10]));
+        b[4] = fmaxf(v[0], fmaxf(v[4], v[8]));
+        ymin = fminf(ymin, b[0]);
+        ymax = fmaxf(ymax, b[1]);
+        zmin = fminf(zmin, b[2]);
+        zmax = fmaxf(zmax, b[3]);
+    }
+    int nb = 1;
+    while (nb * nb < ntri / 4 && nb < 256) nb *= 2;
+    m->by = m->bz = nb;
+    m->y0 = ntri > 0 ? ymin : 0.0f;
+    m->z0 = ntri > 0 ? zmin : 0.0f;
+    float ey = ntri > 0 ? ymax - ymin : 1.0f, ez = ntri > 0 ? zmax - zmin : 1.0f;
+    m->iy = ey > 0 ? nb / ey : 1.0f;
+    m->iz = ez > 0 ? nb / ez : 1.0f;
+    int nbins = nb * nb;
+    m->binStart = (int *)calloc((size_t)nbins + 1, sizeof(int));
+    if (!m->binStart) return 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int *fill = NULL;
+        if (pass == 1) {
+            for (int k = 0; k < nbins; k++) m->binStart[k + 1] += m->binStart[k];
+            m->binList = (int *)malloc(sizeof(int) * (size_t)(m->binStart[nbins] > 0 ? m->binStart[nbins] : 1));
+            fill = (int *)calloc((size_t)nbins, sizeof(int));
+            if (!m->binList || !fill) {
+                free(fill);
+                return 0;
+            }
+        }
+        for (int t = 0; t < ntri; t++) {
+            const float *b = m->bounds + 5L * t;
+            int ya = bin_of(b[0], m->y0, m->iy, nb), yb = bin_of(b[1], m->y0, m->iy, nb);
+            int za = bin_of(b[2], m->z0, m->iz, nb), zb = bin_of(b[3], m->z0, m->iz, nb);
+            for (int zz = za; zz <= zb; zz++)
+                for (int yy = ya; yy <= yb; yy++) {
+                    int k = yy + nb * zz;
+                    if (pass == 0)
+                        m->binStart[k + 1]++;
+                    else
+                        m->binList[m->binStart[k] + fill[k]++] = t;
+                }
+        }
+        free(fill);
+    }
+    return 1;
+}
+
+static void mesh_index_free(MeshIndex *m) {
+    free(m->bounds);
+    free(m->binStart);
+    free(m->binList);
+}
+
+/* Ray-triangle test (Moller & Trumbore 1997) in the float expression order
+ * the reference uses; returns the ray parameter through *t.  The +X solid
+ * test specialises h = d x e2 and v = f (d . q) for d = (1,0,0). */
+static int ray_tri(const float *v, float ox, float oy, float oz, float dx, float dy, float dz, int plusX,
+                   float eps, float *t) {
+    float e1x = v[4] - v[0], e1y = v[5] - v[1], e1z = v[6] - v[2];
+    float e2x = v[8] - v[0], e2y = v[9] - v[1], e2z = v[10] - v[2];
+    float hx, hy, hz;
+    if (plusX) {
+        hx = 0;
+        hy = -e2z;
+        hz = e2y;
+    } else {
+        hx = dy * e2z - dz * e2y;
+        hy = dz * e2x - dx * e2z;
+        hz = dx * e2y - dy * e2x;
+    }
+    float a = e1x * hx + e1y * hy + e1z * hz;
+    if (a > -eps && a < eps) return 0;
+    float f = 1.0f / a;
+    float sx = ox - v[0], sy = oy - v[1], sz = oz - v[2];
+    float u = f * (sx * hx + sy * hy + sz * hz);
+    if (u < 0.0f || u > 1.0f) return 0;
+    float qx = sy * e1z - sz * e1y;
+    float qy = sz * e1x - sx * e1z;
+    float qz = sx * e1y - sy * e1x;
+    float vv = plusX ? f * qx : f * (dx * qx + dy * qy + dz * qz);
+    if (vv < 0.0f || u + vv > 1.0f) return 0;
+    *t = f * (e2x * qx + e2y * qy + e2z * qz);
+    return 1;
+}
+
+/* Candidate triangles for a cell centre: the bin's list filtered by the
+ * reference's cull test. */
+#define FOR_CANDIDATES(m, wx, wy, wz, T)                                                        \
+    for (int _k = (m)->binStart[bin_of(wy, (m)->y0, (m)->iy, (m)->by) +                       \
+                                (m)->by * bin_of(wz, (m)->z0, (m)->iz, (m)->bz)],              \
+             _e = (m)->binStart[bin_of(wy, (m)->y0, (m)->iy, (m)->by) +                        \
+                                (m)->by * bin_of(wz, (m)->z0, (m)->iz, (m)->bz) + 1];          \
+         _k < _e; _k++)                                                                         \
+        if ((T = (m)->binList[_k]), !((wy) < (m)->bounds[5L * T + 0] || (wy) > (m)->bounds[5L * T + 1] || \
+                                       (wz) < (m)->bounds[5L * T + 2] || (wz) > (m)->bounds[5L * T + 3] || \
+                                       (wx) > (m)->bounds[5L * T + 4]))
+
+typedef struct {
+    Scale s;
+    MeshIndex *m;
+} MeshCtx;
+
+static float mesh_q(void *vctx, int x, int y, int z, int i) {
+    MeshCtx *c = (MeshCtx *)vctx;
+    float wx, wy, wz;
+    cell_centre(c->s, x, y, z, &wx, &wy, &wz);
+    float dx = (float)kEx[i] / c->s.sx, dy = (float)kEy[i] / c->s.sy, dz = (float)kEz[i] / c->s.sz;
+    float best = 2.0f;
+    int t;
+    if (c->m->ntri > 0) {
+        FOR_CANDIDATES(c->m, wx, wy, wz, t) {
+            float th;
+            if (ray_tri(c->m->tri + 12L * t, wx, wy, wz, dx, dy, dz, 0, 1e-6f, &th) && th > 0.0f && th < best)
+                best = th;
+        }
+    }
+    return clamp_q(best <= 1.0f ? best : 0.5f);
+}
+
+/* simulation/src/lbm.c:964-1223 */
+void LBM_SetSolidMesh(LBMGrid *g, float *triangles, int numTriangles, float minX, float minY, float minZ,
+                      float maxX, float maxY, float maxZ) {
+    if (!g) return;
+    Impl *im = impl_of(g);
+    MeshIndex m;
+    if (!mesh_index_build(&m, triangles, numTriangles < 0 ? 0 : numTriangles)) {
+        mesh_index_free(&m);
+        note_error("LBM_SetSolidMesh: host allocation failed");
+        return;
+    }
+    Scale s = world_scale(g);
+    memset(im->solid, 0, sizeof(int32_t) * (size_t)g->totalCells);
+    int count = 0;
+
+    for (int z = 0; z < g->sizeZ; z++) {
+        for (int y = 0; y < g->sizeY; y++)
+            for (int x = 0; x < g->sizeX; x++) {
+                float wx, wy, wz;
+                cell_centre(s, x, y, z, &wx, &wy, &wz);
+                if (wx < minX || wx > maxX || wy < minY || wy > maxY || wz < minZ || wz > maxZ) continue;
+                if (m.ntri == 0) continue;
+                int hits = 0, t;
+                FOR_CANDIDATES(&m, wx, wy, wz, t) {
+                    float d;
+                    if (ray_tri(triangles + 12L * t, wx, wy, wz, 1.0f, 0.0f, 0.0f, 1, 0.00001f, &d) &&
+                        d > 0.00001f)
+                        hits++;
+                }
+                if (hits % 2 == 1) {
+                    im->solid[cell_index(g, x, y, z)] = 1;
+                    count++;
+                }
+            }
+    }
+    printf("LBM mesh solid: %d cells marked as solid\n", count);
+    q_reset(im, g->totalCells);
+    MeshCtx ctx = {s, &m};
+    mark_links(g, not_body, not_fluid, mesh_q, &ctx);
+    printf("Bouzidi: %d boundary cells with links\n", im->nRows);
+    mesh_index_free(&m);
+    geometry_changed(g);
+}
+
+/* ------------------------------------------------------------ device sync */
+
+/* Builds the boundary list for the current periodicYZ and uploads geometry. */
+static int ensure_geometry(LBMGrid *g) {
+    Impl *im = impl_of(g);
+    const int periodic = g->periodicYZ == 1;
+    if (!im->geomDirty && im->builtPeriodic == periodic) return 1;
+    const long nc = g->totalCells;
+    int n = 0;
+    for (long c = 0; c < nc; c++) {
+        int x = (int)(c % g->sizeX);
+        long t = c / g->sizeX;
+        int y = (int)(t % g->sizeY), z = (int)(t / g->sizeY);
+        int face = !periodic && (y == 0 || y == g->sizeY - 1 || z == 0 || z == g->sizeZ - 1);
+        int xbody = im->solid[c] == 1 && (x == 0 || x == g->sizeX - 1);
+        if (im->qRow[c] >= 0 || face || xbody) n++;
+    }
+    int32_t *cells = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    float *q = (float *)malloc(sizeof(float) * 19 * (size_t)(n > 0 ? n : 1));
+    if (!cells || !q) {
+        free(cells);
+        free(q);
+        note_error("boundary list: host allocation failed");
+        return 0;
+    }
+    int k = 0;
+    for (long c = 0; c < nc; c++) {
+        int x = (int)(c % g->sizeX);
+        long t = c / g->sizeX;
+        int y = (int)(t % g->sizeY), z = (int)(t / g->sizeY);
+        int face = !periodic && (y == 0 || y == g->sizeY - 1 || z == 0 || z == g->sizeZ - 1);
+        int xbody = im->solid[c] == 1 && (x == 0 || x == g->sizeX - 1);
+        if (!(im->qRow[c] >= 0 || face || xbody)) continue;
+        cells[k] = (int32_t)c;
+        for (int i = 0; i < 19; i++) q[(long)i * n + k] = q_get(im, c, i);
+        k++;
+    }
+    int ok = lbmdev_set_geometry(im->dev, im->solid, n, cells, q);
+    free(cells);
+    free(q);
+    if (!ok) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    im->nBoundary = n;
+    im->geomDirty = 0;
+    im->builtPeriodic = periodic;
+    return 1;
+}
+
+int LBM_GetBoundaryCellCount(LBMGrid *g) {
+    if (!g || !ensure_geometry(g)) return -1;
+    return impl_of(g)->nBoundary;
+}
+
+/* ------------------------------------------------------------ flow */
+
+int LBM_InitializeFlow(LBMGrid *g, float ux, float uy, float uz) {
+    if (!g) return 0;
+    Impl *im = impl_of(g);
+    if (g->populationStorage != lbmdev_storage(im->dev)) {
+        char err[256] = {0};
+        lbmdev_destroy(im->dev);
+        im->dev = lbmdev_create(g->sizeX, g->sizeY, g->sizeZ, g->populationStorage, err, sizeof(err));
+        if (!im->dev) {
+            char msg[400];
+            snprintf(msg, sizeof(msg), "LBM_InitializeFlow: %s", err);
+            note_error(msg);
+            /* fall back to a usable FP32 grid so later calls stay valid */
+            g->populationStorage = LBM_STORAGE_FP32;
+            im->dev = lbmdev_create(g->sizeX, g->sizeY, g->sizeZ, LBM_STORAGE_FP32, err, sizeof(err));
+            if (im->dev) {
+                g->velocityBuffer = lbmdev_velocity_ptr(im->dev);
+                g->solidBuffer = lbmdev_solid_ptr(im->dev);
+            }
+            im->geomDirty = 1;
+            return 0;
+        }
+        g->velocityBuffer = lbmdev_velocity_ptr(im->dev);
+        g->solidBuffer = lbmdev_solid_ptr(im->dev);
+        im->geomDirty = 1;
+    }
+    if (!ensure_geometry(g)) return 0;
+    float feq[19], v4[4] = {ux, uy, uz, 1.0f};
+    for (int i = 0; i < 19; i++) feq[i] = host_feq(i, 1.0f, ux, uy, uz);
+    if (!lbmdev_init_flow(im->dev, feq, v4)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    im->flowReady = 1;
+    g->stepCount = 0;
+    printf("LBM flow initialized: u=(%.3f, %.3f, %.3f)\n", ux, uy, uz);
+    return 1;
+}
+
+static void step_params(const LBMGrid *g, float ux, float uy, float uz, LbmStepParams *p) {
+    p->tau = g->tau;
+    p->smagorinskyCs = g->smagorinskyCs;
+    p->inletX = ux;
+    p->inletY = uy;
+    p->inletZ = uz;
+    p->useRegularized = g->useRegularized;
+    p->useMRT = g->useMRT;
+    p->useSmagorinsky = g->useSmagorinsky;
+    p->periodicYZ = g->periodicYZ;
+}
+
+void LBM_StepN(LBMGrid *g, int n, float ux, float uy, float uz) {
+    if (!g || n <= 0) return;
+    Impl *im = impl_of(g);
+    if (!im->flowReady) {
+        note_error("LBM_Step called before LBM_InitializeFlow");
+        return;
+    }
+    if (!ensure_geometry(g)) return;
+    LbmStepParams p;
+    step_params(g, ux, uy, uz, &p);
+    if (!lbmdev_step(im->dev, &p, n)) {
+        note_error(lbmdev_last_error());
+        return;
+    }
+    g->stepCount += n;
+}
+
+void LBM_Step(LBMGrid *g, float ux, float uy, float uz) { LBM_StepN(g, 1, ux, uy, uz); }
+
+int LBM_ProfileSteps(LBMGrid *g, int n, float ux, float uy, float uz, float *total_ms, float *main_ms,
+                     float *boundary_ms) {
+    if (!g || n <= 0) return 0;
+    Impl *im = impl_of(g);
+    if (!im->flowReady || !ensure_geometry(g)) return 0;
+    LbmStepParams p;
+    step_params(g, ux, uy, uz, &p);
+    int ok = lbmdev_profile_steps(im->dev, &p, n, total_ms, main_ms, boundary_ms);
+    if (ok) g->stepCount += n;
+    return ok;
+}
+
+float *LBM_GetVelocityBuffer(LBMGrid *g) { return g ? g->velocityBuffer : NULL; }
+
+void LBM_Synchronize(LBMGrid *g) {
+    if (g) lbmdev_synchronize(impl_of(g)->dev);
+}
+
+/* ------------------------------------------------------------ force */
+
+static int force_sums(LBMGrid *g, double exact[6], long long fixed[7]) {
+    for (int k = 0; k < 6; k++) exact[k] = 0;
+    for (int k = 0; k < 7; k++) fixed[k] = 0;
+    Impl *im = impl_of(g);
+    if (!im->flowReady || !ensure_geometry(g)) return 0;
+    if (!lbmdev_force(im->dev, g->periodicYZ == 1, exact, fixed)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    return 1;
+}
+
+/* simulation/src/lbm.c:835-918 */
+void LBM_ComputeDragForceDecomposed(LBMGrid *g, float *fx, float *fy, float *fz, float *px, float *py,
+                                    float *pz) {
+    *fx = *fy = *fz = 0.0f;
+    if (px) *px = 0.0f;
+    if (py) *py = 0.0f;
+    if (pz) *pz = 0.0f;
+    if (!g) return;
+    double ex[6];
+    long long fi[7];
+    if (!force_sums(g, ex, fi)) return;
+    float out[6];
+    if (g->forceFixedPoint) {
+        /* results[k] / 10000.0f on the int32 accumulators, lbm.c:909-917 */
+        const int src[6] = {0, 1, 2, 4, 5, 6};
+        for (int k = 0; k < 6; k++) out[k] = (float)(int32_t)fi[src[k]] / 10000.0f;
+    } else {
+        for (int k = 0; k < 6; k++) out[k] = (float)ex[k];
+    }
+    *fx = out[0];
+    *fy = out[1];
+    *fz = out[2];
+    if (px) *px = out[3];
+    if (py) *py = out[4];
+    if (pz) *pz = out[5];
+}
+
+void LBM_ComputeDragForce(LBMGrid *g, float *fx, float *fy, float *fz) {
+    LBM_ComputeDragForceDecomposed(g, fx, fy, fz, NULL, NULL, NULL);
+}
+
+void LBM_ComputeDragForceF64(LBMGrid *g, double out[6]) {
+    long long fi[7];
+    for (int k = 0; k < 6; k++) out[k] = 0;
+    if (g) force_sums(g, out, fi);
+}
+
+/* |F| / (0.5 rho U^2 A) with rho = 1; 0 when the denominator is below 1e-10
+ * (simulation/src/lbm.c:920-962). */
+static float coefficient(float force, float U, float A) {
+    float dyn = 0.5f * 1.0f * U * U;
+    if (dyn * A < 1e-10f) return 0.0f;
+    return fabsf(force) / (dyn * A);
+}
+
+float LBM_ComputeDragCoefficient(LBMGrid *g, float U, float A) {
+    float fx, fy, fz;
+    LBM_ComputeDragForce(g, &fx, &fy, &fz);
+    return coefficient(fx, U, A);
+}
+
+float LBM_ComputeLiftCoefficient(LBMGrid *g, float U, float A) {
+    float fx, fy, fz;
+    LBM_ComputeDragForce(g, &fx, &fy, &fz);
+    return coefficient(fy, U, A);
+}
+
+/* ------------------------------------------------------------ readback */
+
+int LBM_CopyVelocityToHost(LBMGrid *g, float *out) {
+    if (!g || !out) return 0;
+    return lbmdev_copy_velocity(impl_of(g)->dev, out);
+}
+
+int LBM_CopySolidToHost(LBMGrid *g, int *out) {
+    if (!g || !out || !ensure_geometry(g)) return 0;
+    return lbmdev_copy_solid(impl_of(g)->dev, (int32_t *)out);
+}
+
+int LBM_CopyPopulationsToHost(LBMGrid *g, float *f, float *fNew) {
+    if (!g || !impl_of(g)->flowReady || !ensure_geometry(g)) return 0;
+    return lbmdev_export_populations(impl_of(g)->dev, g->periodicYZ == 1, f, fNew);
+}
+
+int lbmdev_export_populations_f64(LbmDev *d, int periodicYZ, double *f_ps, double *f_pc);
+
+int LBM_CopyPopulationsToHostF64(LBMGrid *g, double *f, double *fNew) {
+    if (!g || !impl_of(g)->flowReady || !ensure_geometry(g)) return 0;
+    return lbmdev_export_populations_f64(impl_of(g)->dev, g->periodicYZ == 1, f, fNew);
+}
+
+double LBM_ComputeTotalMass(LBMGrid *g) {
+    double m = 0.0;
+    if (!g || !impl_of(g)->flowReady || !ensure_geometry(g)) return 0.0;
+    lbmdev_total_mass(impl_of(g)->dev, g->periodicYZ == 1, &m);
+    return m;
+}
+
+/* ------------------------------------------------------------ particles */
+
+void LBM_SeedParticles(LBMGrid *g, Particle *p, int n, unsigned int seed) {
+    if (!g || !p) return;
+    if (!lbmdev_particles_seed(impl_of(g)->dev, p, n, seed)) note_error(lbmdev_last_error());
+}
+
+void LBM_AdvectParticles(LBMGrid *g, Particle *p, int n, float dt, float velScale, unsigned int seed,
+                         unsigned int step) {
+    if (!g || !p) return;
+    if (!lbmdev_particles_advect(impl_of(g)->dev, p, n, dt, velScale, seed, step))
+        note_error(lbmdev_last_error());
+}
+
+void *LBM_DeviceAlloc(unsigned long long bytes) { return lbmdev_malloc((size_t)bytes); }
+void LBM_DeviceFree(void *p) { lbmdev_free(p); }
+int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes) {
+    return lbmdev_memcpy_h2d(dst, src, (size_t)bytes);
+}
+int LBM_CopyToHost(void *dst, const void *src, unsigned long long bytes) {
+    return lbmdev_memcpy_d2h(dst, src, (size_t)bytes);
+}

@@ -1,0 +1,87 @@
+/*
+ * Thin C-ABI between the C host library (lbm.c) and the HIP kernels
+ * (lbm_kernels.hip).  Plain pointers and sizes only; the host side never
+ * includes a HIP header.  All calls are ordered on one HIP stream per grid.
+ *
+ * Population storage (device): one buffer of 19 direction planes,
+ * plane i at pop + i*stride, cell = x + NX*(y + NY*z).  Updated in place by
+ * the two-phase (esoteric-pull) scheme; see lbm_kernels.hip.
+ */
+#ifndef LBM_DEVICE_H
+#define LBM_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* population storage formats */
+enum { LBMDEV_FP32 = 0, LBMDEV_FP16S = 1, LBMDEV_BF16S = 2, LBMDEV_FP64 = 3 };
+
+/* mask byte bits */
+enum { LBMDEV_MASK_BODY = 1, LBMDEV_MASK_GROUND = 2, LBMDEV_MASK_OVR = 4 };
+
+typedef struct LbmDev LbmDev;
+
+typedef struct {
+    float tau;
+    float smagorinskyCs;
+    float inletX, inletY, inletZ;
+    int useRegularized, useMRT, useSmagorinsky, periodicYZ;
+} LbmStepParams;
+
+LbmDev *lbmdev_create(int nx, int ny, int nz, int storage, char *err, int errlen);
+void lbmdev_destroy(LbmDev *d);
+int lbmdev_storage(const LbmDev *d);
+const char *lbmdev_last_error(void);
+
+/* solid: int per cell (0/1/2).  ovr_cells: sorted cell indices of the
+ * boundary cells whose incoming populations are computed by the stream-rule
+ * prepass; ovr_q: q[i*n_ovr + k] for those cells (-1 = no link). */
+int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, const int32_t *ovr_cells,
+                        const float *ovr_q);
+
+/* f = f_new = feq (19 host-computed values) everywhere, vel = vel4. */
+int lbmdev_init_flow(LbmDev *d, const float feq[19], const float vel4[4]);
+
+/* n LBM steps; the velocity buffer is written on the last one. */
+int lbmdev_step(LbmDev *d, const LbmStepParams *p, int nsteps);
+
+/* Momentum exchange over the boundary links (periodicYZ selects the stream
+ * rule).  exact[6] = FP64 sums (F xyz, P xyz); fixed[7] = reference fixed
+ * point accumulators (round(F*1e4) xyz, link-cell count, round(P*1e4) xyz). */
+int lbmdev_force(LbmDev *d, int periodicYZ, double exact[6], long long fixed[7]);
+
+float *lbmdev_velocity_ptr(LbmDev *d);
+int32_t *lbmdev_solid_ptr(LbmDev *d);
+int lbmdev_copy_velocity(LbmDev *d, float *host);
+int lbmdev_copy_solid(LbmDev *d, int32_t *host);
+/* Reference-layout views (f[i + 19*cell]): post-stream f and post-collision
+ * f_new of the last step, as FP32 (either may be NULL). */
+int lbmdev_export_populations(LbmDev *d, int periodicYZ, float *f_post_stream, float *f_post_collision);
+/* Sum of rho over all cells in FP64 (mass check) from the post-stream state. */
+int lbmdev_total_mass(LbmDev *d, int periodicYZ, double *mass);
+
+int lbmdev_particles_seed(LbmDev *d, void *dev_particles, int n, uint32_t seed);
+int lbmdev_particles_advect(LbmDev *d, void *dev_particles, int n, float dt, float velScale,
+                            uint32_t seed, uint32_t step);
+
+int lbmdev_synchronize(LbmDev *d);
+/* Times `nsteps` steps with HIP events on the grid's stream.  total_ms is the
+ * whole region; main_ms / pre_ms are sums of per-launch event pairs around
+ * the fused step kernel and the boundary prepass (either may be NULL). */
+int lbmdev_profile_steps(LbmDev *d, const LbmStepParams *p, int nsteps, float *total_ms,
+                         float *main_ms, float *pre_ms);
+
+void *lbmdev_malloc(size_t bytes);
+void lbmdev_free(void *p);
+int lbmdev_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int lbmdev_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int lbmdev_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
