@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3Q19 step on one MI355X.
+
+Workload (default): the 512x256x256 roofline configuration -- regularized
+collision + Smagorinsky (Cs 0.1), FP32 populations, tau 0.505, inlet u 0.05,
+periodic y/z, a seeded (seed 0) random ellipsoid-plus-box car proxy meshed
+with ~20k triangles and Bouzidi bounce-back, 50,000 particles seeded on the
+inlet plane from seed 0 and advected with RK2 trilinear sampling every 10
+LBM steps (the substep count of the reference's main loop).
+
+A "step" is one LBM update of the whole lattice.  `value` is lattice updates
+per second of the whole timed region (inputs resident in HBM; includes the
+boundary prepass, the particle kernel and the host synchronisations).
+`roofline` prices the fused step kernel alone at 152 B/cell/step (19 loads +
+19 stores of FP32) against its average launch time from HIP events recorded on
+the grid's stream around every launch.  `cpu_baseline` times the oracle (the
+C/OpenMP restatement of the reference shaders) on a bounded sample of the same
+workload on the host cores.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {0: 152.0, 1: 76.0, 2: 76.0, 3: 304.0}
+
+WORKLOADS = {
+    "roofline-512x256x256-regularized-smagorinsky": dict(
+        dims=(512, 256, 256), op="reg", smag=1, tau=0.505, u=0.05, particles=50000, storage=0),
+    "512x256x256-mrt-smagorinsky": dict(
+        dims=(512, 256, 256), op="mrt", smag=1, tau=0.505, u=0.05, particles=50000, storage=0),
+    "256x128x128-mrt-smagorinsky-fp16": dict(
+        dims=(256, 128, 128), op="mrt", smag=1, tau=0.52, u=0.06, particles=0, storage=1),
+}
+DEFAULT = "roofline-512x256x256-regularized-smagorinsky"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup_grid(lbm, geometry, w):
+    nx, ny, nz = w["dims"]
+    g = lbm.LBM_Create(nx, ny, nz, (w["tau"] - 0.5) / 3)
+    g.tau = w["tau"]
+    g.useRegularized = int(w["op"] == "reg")
+    g.useMRT = int(w["op"] == "mrt")
+    g.useSmagorinsky = w["smag"]
+    g.smagorinskyCs = 0.1
+    g.periodicYZ = 1
+    g.populationStorage = w["storage"]
+    t0 = time.time()
+    tris, bbox = geometry.car_proxy(seed=0)
+    lbm.LBM_SetSolidMesh(g, tris, bbox)
+    area = lbm.LBM_ComputeProjectedArea(g, 0)
+    if lbm.LBM_InitializeFlow(g, w["u"], 0.0, 0.0) != 1:
+        raise RuntimeError("LBM_InitializeFlow failed: " + lbm.last_error())
+    log(f"setup {time.time() - t0:.1f}s: {len(tris)} triangles, area {area}, "
+        f"boundary cells {lbm.LBM_GetBoundaryCellCount(g)}")
+    return g, tris, bbox, area
+
+
+def run_frames(lbm, g, parts, w, nsteps, step0, per_launch=True):
+    """nsteps LBM steps in frames of 10 (particles advected after each frame).
+    Returns (wall_s, main_ms_sum, boundary_ms_sum, region_ms_sum)."""
+    nx = w["dims"][0]
+    vel_scale = 8.0 / nx
+    main = bnd = region = 0.0
+    lbm.LBM_Synchronize(g)
+    t0 = time.perf_counter()
+    done = 0
+    frame = 0
+    while done < nsteps:
+        n = min(10, nsteps - done)
+        tot, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=per_launch)
+        main += mn
+        bnd += bd
+        region += tot
+        done += n
+        frame += 1
+        if parts is not None:
+            lbm.LBM_AdvectParticles(g, parts, float(n), vel_scale, 0, step0 + frame)
+    lbm.LBM_Synchronize(g)
+    return time.perf_counter() - t0, main, bnd, region
+
+
+def cpu_baseline(lbm, g, w, steps):
+    """The FP32 oracle on the same lattice, geometry and operator (solid and q
+    copied from the product), timed over `steps` steps on the host cores."""
+    from oracle import oracle as O
+
+    nx, ny, nz = w["dims"]
+    og = O.OracleGrid(nx, ny, nz, (w["tau"] - 0.5) / 3, precision=32)
+    og.tau = float(np.float32(w["tau"]))
+    og.useRegularized = int(w["op"] == "reg")
+    og.useMRT = int(w["op"] == "mrt")
+    og.useSmagorinsky = w["smag"]
+    og.smagorinskyCs = 0.1
+    og.periodicYZ = 1
+    og.solid[:] = lbm.LBM_CopySolidToHost(g)
+    og.q[:] = lbm.LBM_CopyQToHost(g)
+    og.initialize_flow(w["u"], 0.0, 0.0)
+    og.step(w["u"], 0, 0, n=1)  # first touch of the buffers, untimed
+    t0 = time.perf_counter()
+    og.step(w["u"], 0, 0, n=steps)
+    dt = time.perf_counter() - t0
+    cells = nx * ny * nz
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the {nx}x{ny}x{nz} {w['op']}+Smagorinsky workload "
+                      f"({dt:.1f} s), oracle C/OpenMP restatement of the reference shaders, FP32, AoS two-buffer"}
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3 PMC
+    summary for this workload, if present."""
+    p = os.path.join(ROOT, "profiles", f"pmc_step_{workload}.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--no-per-launch", action="store_true", help="skip the per-launch kernel events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
+              "decomposition is a design note, not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from lattice_fluid_dynamics_amd import geometry, lbm
+
+    w = WORKLOADS[args.workload]
+    g, tris, bbox, area = setup_grid(lbm, geometry, w)
+    nx, ny, nz = w["dims"]
+    cells = nx * ny * nz
+    parts = None
+    if w["particles"]:
+        parts = lbm.DeviceParticles(w["particles"])
+        lbm.LBM_SeedParticles(g, parts, 0)
+
+    if args.warmup:
+        run_frames(lbm, g, parts, w, args.warmup, 0, per_launch=False)
+    wall, main_ms, bnd_ms, region_ms = run_frames(lbm, g, parts, w, args.steps, 10 ** 6,
+                                                   per_launch=not args.no_per_launch)
+    mlups = cells * args.steps / wall / 1e6
+    fx, fy, fz, px, py, pz = lbm.LBM_ComputeDragForceDecomposed(g)
+    dyn = 0.5 * w["u"] ** 2 * area
+    sane = np.isfinite(fx) and lbm.LBM_ComputeTotalMass(g) > 0.5 * cells
+    if not sane:
+        raise RuntimeError("simulation state is not finite after the timed region")
+
+    bpc = BYTES_PER_CELL[w["storage"]]
+    roof = None
+    if main_ms > 0:
+        avg_s = main_ms / args.steps / 1e3
+        achieved = bpc * cells / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload),
+                "kernel": "k_step (fused stream+collide)", "bytes_per_cell": bpc,
+                "avg_launch_ms": round(main_ms / args.steps, 5),
+                "boundary_prepass_avg_ms": round(bnd_ms / max(args.steps, 1), 5)}
+    cpu = None
+    if args.cpu_steps > 0:
+        log("timing the CPU oracle ...")
+        cpu = cpu_baseline(lbm, g, w, args.cpu_steps)
+
+    storage = {0: "f32", 1: "f16 (shifted storage, f32 math)", 2: "bf16 (shifted storage, f32 math)", 3: "f64"}
+    out = {
+        "metric": "MLUPS (D3Q19 lattice updates per second; % of HBM roofline at 152 B/cell/step FP32)",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": storage[w["storage"]], "data": "synthetic",
+        "config": {"workload": args.workload, "lattice": f"{nx}x{ny}x{nz}",
+                   "collision": {"reg": "regularized", "mrt": "MRT"}[w["op"]] + "+Smagorinsky(Cs=0.1)",
+                   "tau": w["tau"], "inlet_u": w["u"], "periodicYZ": 1,
+                   "body": f"car proxy seed 0, {len(tris)} triangles, Bouzidi",
+                   "particles": w["particles"], "particle_interval_steps": 10, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "cd": round(abs(fx) / dyn, 4) if dyn > 0 else None,
+        "hbm_roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpc / 1e6, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

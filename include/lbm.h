@@ -60,8 +60,9 @@ typedef struct {
     int haloZ;
 
     /* Additions (no reference counterpart). */
-    int forceFixedPoint;   /* 1: accumulate the force as the reference does, round(F*1e4)
-                              per boundary cell in integers; 0 (default): FP64 sum */
+    int forceFixedPoint;   /* 1 (default): accumulate the force as the reference does,
+                              round(F*1e4) per boundary cell in integers; 0: FP64 sum of
+                              the per-cell forces (LBM_ComputeDragForceF64 gives both) */
     int populationStorage; /* LBM_STORAGE_*; takes effect at LBM_InitializeFlow */
     long long stepCount;   /* LBM_Step calls since LBM_InitializeFlow */
     void *impl;            /* opaque */
@@ -130,6 +131,10 @@ int LBM_CopyVelocityToHost(LBMGrid *grid, float *out);
 
 /* Replaces glGetBufferSubData on solidBuffer (test_lbm.c:258-260). */
 int LBM_CopySolidToHost(LBMGrid *grid, int *out);
+
+/* Replaces glGetBufferSubData on the Bouzidi q buffer (simulation/src/lbm.c:
+ * 233-250 layout): q[i + 19*cell], -1 where cell has no link in direction i. */
+int LBM_CopyQToHost(LBMGrid *grid, float *out);
 
 /* The reference's two population buffers in their layout f[i + 19*cell]:
  * post-stream f and post-collision f_new of the last step (either may be NULL). */

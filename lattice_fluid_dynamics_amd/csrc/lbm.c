@@ -798,6 +798,14 @@ int LBM_CopySolidToHost(LBMGrid *g, int *out) {
     return lbmdev_copy_solid(impl_of(g)->dev, (int32_t *)out);
 }
 
+int LBM_CopyQToHost(LBMGrid *g, float *out) {
+    if (!g || !out) return 0;
+    Impl *im = impl_of(g);
+    for (long c = 0; c < g->totalCells; c++)
+        for (int i = 0; i < 19; i++) out[19 * c + i] = q_get(im, c, i);
+    return 1;
+}
+
 int LBM_CopyPopulationsToHost(LBMGrid *g, float *f, float *fNew) {
     if (!g || !impl_of(g)->flowReady || !ensure_geometry(g)) return 0;
     return lbmdev_export_populations(impl_of(g)->dev, g->periodicYZ == 1, f, fNew);

@@ -1,0 +1,324 @@
+"""Python binding of the C ABI in include/lbm.h (liblbm_amd.so).
+
+The functions keep the names and argument order of the reference solver API
+(MarcosAsh/Lattice_Fluid_Dynamics simulation/lib/lbm.h:58-129) so host code
+and tests read like the reference's own: ``g = LBM_Create(128, 64, 64, nu)``,
+``g.tau = 0.6``, ``LBM_Step(g, 0.05, 0, 0)``, ``LBM_ComputeDragForce(g)``.
+Out-pointer arguments of the C API become return values.
+
+The library is loaded from this package directory.  There is no fallback:
+if the shared library is missing, or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm_amd.so")
+
+LBM_STORAGE_FP32 = 0
+LBM_STORAGE_FP16 = 1
+LBM_STORAGE_BF16 = 2
+LBM_STORAGE_FP64 = 3
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+class LBMGridStruct(ctypes.Structure):
+    """Field-for-field mirror of LBMGrid in include/lbm.h."""
+
+    _fields_ = [
+        ("sizeX", c_int), ("sizeY", c_int), ("sizeZ", c_int), ("totalCells", c_int),
+        ("tau", c_float),
+        ("velocityBuffer", c_void_p), ("solidBuffer", c_void_p),
+        ("useRegularized", c_int), ("useMRT", c_int), ("useSmagorinsky", c_int),
+        ("smagorinskyCs", c_float), ("periodicYZ", c_int),
+        ("numChunks", c_int), ("slabZ", c_int), ("haloZ", c_int),
+        ("forceFixedPoint", c_int), ("populationStorage", c_int),
+        ("stepCount", ctypes.c_longlong), ("impl", c_void_p),
+    ]
+
+
+GridP = ctypes.POINTER(LBMGridStruct)
+
+# Every symbol include/lbm.h declares, with its ctypes signature.
+SIGNATURES = {
+    "LBM_Create": (GridP, [c_int, c_int, c_int, c_float]),
+    "LBM_Free": (None, [GridP]),
+    "LBM_SetSolidAABB": (None, [GridP] + [c_float] * 6),
+    "LBM_InitializeFlow": (c_int, [GridP, c_float, c_float, c_float]),
+    "LBM_Step": (None, [GridP, c_float, c_float, c_float]),
+    "LBM_GetVelocityBuffer": (c_void_p, [GridP]),
+    "LBM_ComputeDragForce": (None, [GridP, c_float_p, c_float_p, c_float_p]),
+    "LBM_ComputeDragForceDecomposed": (None, [GridP] + [c_float_p] * 6),
+    "LBM_ComputeDragCoefficient": (c_float, [GridP, c_float, c_float]),
+    "LBM_ComputeLiftCoefficient": (c_float, [GridP, c_float, c_float]),
+    "LBM_SetSolidSphere": (None, [GridP, c_float, c_float, c_float, c_float]),
+    "LBM_AddGroundPlane": (None, [GridP, c_float]),
+    "LBM_ComputeProjectedArea": (c_float, [GridP, c_int]),
+    "LBM_SetSolidMesh": (None, [GridP, c_float_p, c_int] + [c_float] * 6),
+    "LBM_StepN": (None, [GridP, c_int, c_float, c_float, c_float]),
+    "LBM_CopyVelocityToHost": (c_int, [GridP, c_float_p]),
+    "LBM_CopySolidToHost": (c_int, [GridP, c_int_p]),
+    "LBM_CopyQToHost": (c_int, [GridP, c_float_p]),
+    "LBM_CopyPopulationsToHost": (c_int, [GridP, c_float_p, c_float_p]),
+    "LBM_CopyPopulationsToHostF64": (c_int, [GridP, c_double_p, c_double_p]),
+    "LBM_ComputeTotalMass": (c_double, [GridP]),
+    "LBM_ComputeDragForceF64": (None, [GridP, c_double_p]),
+    "LBM_SeedParticles": (None, [GridP, c_void_p, c_int, ctypes.c_uint]),
+    "LBM_AdvectParticles": (None, [GridP, c_void_p, c_int, c_float, c_float, ctypes.c_uint, ctypes.c_uint]),
+    "LBM_DeviceAlloc": (c_void_p, [ctypes.c_ulonglong]),
+    "LBM_DeviceFree": (None, [c_void_p]),
+    "LBM_CopyToDevice": (c_int, [c_void_p, c_void_p, ctypes.c_ulonglong]),
+    "LBM_CopyToHost": (c_int, [c_void_p, c_void_p, ctypes.c_ulonglong]),
+    "LBM_Synchronize": (None, [GridP]),
+    "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
+    "LBM_GetBoundaryCellCount": (c_int, [GridP]),
+    "LBM_GetLastError": (ctypes.c_char_p, []),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads liblbm_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `make -C lattice_fluid_dynamics_amd/csrc`")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return load_library().LBM_GetLastError().decode()
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(c_float_p)
+
+
+class Grid:
+    """Owning handle of an LBMGrid*; struct fields read and write through."""
+
+    def __init__(self, ptr):
+        object.__setattr__(self, "_p", ptr)
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def __getattr__(self, name):
+        return getattr(self._p.contents, name)
+
+    def __setattr__(self, name, value):
+        if name in dict(LBMGridStruct._fields_):
+            setattr(self._p.contents, name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def free(self):
+        if self._p:
+            load_library().LBM_Free(self._p)
+            object.__setattr__(self, "_p", None)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def shape(self) -> Tuple[int, int, int]:
+        return (self.sizeX, self.sizeY, self.sizeZ)
+
+
+# ---- the reference API --------------------------------------------------------
+
+def LBM_Create(sizeX: int, sizeY: int, sizeZ: int, viscosity: float) -> Grid:
+    p = load_library().LBM_Create(sizeX, sizeY, sizeZ, viscosity)
+    if not p:
+        raise RuntimeError(f"LBM_Create failed: {last_error()}")
+    return Grid(p)
+
+
+def LBM_Free(g: Grid) -> None:
+    g.free()
+
+
+def LBM_SetSolidAABB(g: Grid, minX, minY, minZ, maxX, maxY, maxZ) -> None:
+    load_library().LBM_SetSolidAABB(g.ptr, minX, minY, minZ, maxX, maxY, maxZ)
+
+
+def LBM_SetSolidSphere(g: Grid, cx, cy, cz, radius) -> None:
+    load_library().LBM_SetSolidSphere(g.ptr, cx, cy, cz, radius)
+
+
+def LBM_SetSolidMesh(g: Grid, triangles: np.ndarray, bbox) -> None:
+    t = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1)
+    load_library().LBM_SetSolidMesh(g.ptr, _fp(t), len(t) // 12, *[float(v) for v in bbox])
+
+
+def LBM_AddGroundPlane(g: Grid, worldZ: float) -> None:
+    load_library().LBM_AddGroundPlane(g.ptr, worldZ)
+
+
+def LBM_ComputeProjectedArea(g: Grid, axis: int) -> float:
+    return float(load_library().LBM_ComputeProjectedArea(g.ptr, axis))
+
+
+def LBM_InitializeFlow(g: Grid, ux: float, uy: float, uz: float) -> int:
+    return int(load_library().LBM_InitializeFlow(g.ptr, ux, uy, uz))
+
+
+def LBM_Step(g: Grid, ux: float, uy: float, uz: float) -> None:
+    load_library().LBM_Step(g.ptr, ux, uy, uz)
+
+
+def LBM_StepN(g: Grid, n: int, ux: float, uy: float, uz: float) -> None:
+    load_library().LBM_StepN(g.ptr, n, ux, uy, uz)
+
+
+def LBM_GetVelocityBuffer(g: Grid) -> int:
+    return int(load_library().LBM_GetVelocityBuffer(g.ptr) or 0)
+
+
+def LBM_ComputeDragForce(g: Grid):
+    f = [c_float(), c_float(), c_float()]
+    load_library().LBM_ComputeDragForce(g.ptr, *[ctypes.byref(v) for v in f])
+    return tuple(v.value for v in f)
+
+
+def LBM_ComputeDragForceDecomposed(g: Grid):
+    f = [c_float() for _ in range(6)]
+    load_library().LBM_ComputeDragForceDecomposed(g.ptr, *[ctypes.byref(v) for v in f])
+    return tuple(v.value for v in f)
+
+
+def LBM_ComputeDragCoefficient(g: Grid, U: float, A: float) -> float:
+    return float(load_library().LBM_ComputeDragCoefficient(g.ptr, U, A))
+
+
+def LBM_ComputeLiftCoefficient(g: Grid, U: float, A: float) -> float:
+    return float(load_library().LBM_ComputeLiftCoefficient(g.ptr, U, A))
+
+
+# ---- additions ---------------------------------------------------------------
+
+def LBM_CopyVelocityToHost(g: Grid) -> np.ndarray:
+    out = np.empty((g.totalCells, 4), np.float32)
+    if not load_library().LBM_CopyVelocityToHost(g.ptr, _fp(out)):
+        raise RuntimeError(last_error())
+    return out
+
+
+def LBM_CopySolidToHost(g: Grid) -> np.ndarray:
+    out = np.empty(g.totalCells, np.int32)
+    if not load_library().LBM_CopySolidToHost(g.ptr, out.ctypes.data_as(c_int_p)):
+        raise RuntimeError(last_error())
+    return out
+
+
+def LBM_CopyQToHost(g: Grid) -> np.ndarray:
+    out = np.empty((g.totalCells, 19), np.float32)
+    if not load_library().LBM_CopyQToHost(g.ptr, _fp(out)):
+        raise RuntimeError(last_error())
+    return out
+
+
+def LBM_CopyPopulationsToHost(g: Grid, f64: bool = False):
+    """(f post-stream, f_new post-collision) in the reference layout [cell, 19]."""
+    dt = np.float64 if f64 else np.float32
+    f = np.empty((g.totalCells, 19), dt)
+    fn = np.empty((g.totalCells, 19), dt)
+    L = load_library()
+    if f64:
+        ok = L.LBM_CopyPopulationsToHostF64(g.ptr, f.ctypes.data_as(c_double_p), fn.ctypes.data_as(c_double_p))
+    else:
+        ok = L.LBM_CopyPopulationsToHost(g.ptr, _fp(f), _fp(fn))
+    if not ok:
+        raise RuntimeError(last_error())
+    return f, fn
+
+
+def LBM_ComputeTotalMass(g: Grid) -> float:
+    return float(load_library().LBM_ComputeTotalMass(g.ptr))
+
+
+def LBM_ComputeDragForceF64(g: Grid) -> np.ndarray:
+    out = np.zeros(6, np.float64)
+    load_library().LBM_ComputeDragForceF64(g.ptr, out.ctypes.data_as(c_double_p))
+    return out
+
+
+def LBM_Synchronize(g: Grid) -> None:
+    load_library().LBM_Synchronize(g.ptr)
+
+
+def LBM_ProfileSteps(g: Grid, n: int, ux: float, uy: float, uz: float, per_launch: bool = True):
+    tot, mn, bd = c_float(), c_float(), c_float()
+    L = load_library()
+    ok = L.LBM_ProfileSteps(g.ptr, n, ux, uy, uz, ctypes.byref(tot),
+                            ctypes.byref(mn) if per_launch else None, ctypes.byref(bd) if per_launch else None)
+    if not ok:
+        raise RuntimeError(last_error())
+    return tot.value, mn.value, bd.value
+
+
+def LBM_GetBoundaryCellCount(g: Grid) -> int:
+    return int(load_library().LBM_GetBoundaryCellCount(g.ptr))
+
+
+class DeviceParticles:
+    """Device array of n 32-byte Particle records."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.ptr = load_library().LBM_DeviceAlloc(32 * max(n, 1))
+        if not self.ptr:
+            raise MemoryError("device allocation for particles failed")
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=np.float32).reshape(self.n, 8)
+        if not load_library().LBM_CopyToDevice(self.ptr, h.ctypes.data, 32 * self.n):
+            raise RuntimeError(last_error())
+
+    def download(self) -> np.ndarray:
+        h = np.empty((self.n, 8), np.float32)
+        if not load_library().LBM_CopyToHost(h.ctypes.data, self.ptr, 32 * self.n):
+            raise RuntimeError(last_error())
+        return h
+
+    def free(self):
+        if self.ptr:
+            load_library().LBM_DeviceFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def LBM_SeedParticles(g: Grid, parts: DeviceParticles, seed: int) -> None:
+    load_library().LBM_SeedParticles(g.ptr, parts.ptr, parts.n, seed)
+
+
+def LBM_AdvectParticles(g: Grid, parts: DeviceParticles, dt: float, vel_scale: float, seed: int, step: int) -> None:
+    load_library().LBM_AdvectParticles(g.ptr, parts.ptr, parts.n, dt, vel_scale, seed, step)

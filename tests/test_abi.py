@@ -1,0 +1,75 @@
+"""The C ABI: liblbm_amd.so loads and exports every function include/lbm.h
+declares, and the Python mirror of LBMGrid has the C layout.  No GPU calls."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from lattice_fluid_dynamics_amd import lbm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lbm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(LBM_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_reference_api():
+    names = set(declared_functions())
+    reference_api = {  # simulation/lib/lbm.h:58-129
+        "LBM_Create", "LBM_Free", "LBM_SetSolidAABB", "LBM_InitializeFlow", "LBM_Step",
+        "LBM_GetVelocityBuffer", "LBM_ComputeDragForce", "LBM_ComputeDragForceDecomposed",
+        "LBM_ComputeDragCoefficient", "LBM_ComputeLiftCoefficient", "LBM_SetSolidSphere",
+        "LBM_AddGroundPlane", "LBM_ComputeProjectedArea", "LBM_SetSolidMesh"}
+    assert reference_api <= names
+
+
+def test_library_exports_every_declared_symbol():
+    L = lbm.load_library()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    assert set(declared_functions()) == set(lbm.SIGNATURES), \
+        set(declared_functions()) ^ set(lbm.SIGNATURES)
+
+
+def test_grid_struct_layout_matches_c():
+    fields = [f for f, _ in lbm.LBMGridStruct._fields_]
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "lbm.h"\nint main(void){\n'
+    src += 'printf("%zu\\n", sizeof(LBMGrid));\n'
+    for f in fields:
+        src += f'printf("%zu\\n", offsetof(LBMGrid, {f}));\n'
+    src += "return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(lbm.LBMGridStruct)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(lbm.LBMGridStruct, f).offset == off, f
+
+
+def test_particle_record_is_32_bytes():
+    src = '#include <stdio.h>\n#include "lbm.h"\nint main(void){printf("%zu", sizeof(Particle));return 0;}\n'
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        assert subprocess.run([exe], capture_output=True, text=True, check=True).stdout == "32"
+
+
+def test_create_without_device_fails_loudly(monkeypatch):
+    from tests.conftest import gpu_available
+
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError):
+        lbm.LBM_Create(16, 8, 8, 0.1)

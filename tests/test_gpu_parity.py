@@ -1,0 +1,229 @@
+"""Parity of the HIP path (through the C ABI of include/lbm.h) with the CPU
+oracle on the same inputs.
+
+Bars (FP32): after N steps the velocity, density and population fields agree
+with the FP32 oracle to relative L2 <= 1e-5; Cd/Cl within 0.1%; geometry (solid,
+q) and the initial state bit-exact.  FP64 storage agrees with the FP64 oracle
+to relative L2 <= 1e-12.  Every test prints the measured differences.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import C1_BOX, compare, fields, make_pair, rel_l2, setup_aabb, setup_sphere
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def test_grid_create_and_fields(gpu):  # test_lbm.c:235-247
+    g = gpu.LBM_Create(16, 8, 8, 0.1)
+    assert (g.sizeX, g.sizeY, g.sizeZ, g.totalCells) == (16, 8, 8, 1024)
+    assert abs(g.tau - 0.8) < 1e-4
+    assert g.velocityBuffer and g.solidBuffer
+    assert gpu.LBM_GetVelocityBuffer(g) == g.velocityBuffer
+    g2 = gpu.LBM_Create(16, 8, 8, 0.001)
+    assert g2.tau == pytest.approx(0.52)  # clamp, simulation/src/lbm.c:56-60
+
+
+def test_solid_aabb_marks_cells(gpu):  # test_lbm.c:249-270
+    g = gpu.LBM_Create(16, 8, 8, 0.1)
+    gpu.LBM_SetSolidAABB(g, -0.5, -0.25, -0.25, 0.5, 0.25, 0.25)
+    s = gpu.LBM_CopySolidToHost(g)
+    assert 0 < np.count_nonzero(s) < g.totalCells
+
+
+def box_mesh(lo, hi):
+    from tests.test_oracle import box_mesh as bm
+    return bm(lo, hi)
+
+
+@pytest.mark.parametrize("kind", ["aabb", "sphere", "mesh", "ground"])
+def test_geometry_matches_oracle(gpu, kind):
+    nx, ny, nz = 64, 32, 32
+    g = gpu.LBM_Create(nx, ny, nz, 0.05)
+    og = O.OracleGrid(nx, ny, nz, 0.05)
+    if kind == "aabb":
+        gpu.LBM_SetSolidAABB(g, -1.3, -0.6, -0.4, 0.2, 0.5, 0.7)
+        og.set_solid_aabb(-1.3, -0.6, -0.4, 0.2, 0.5, 0.7)
+    elif kind == "sphere":
+        gpu.LBM_SetSolidSphere(g, 0.1, -0.05, 0.03, 0.7)
+        og.set_solid_sphere(0.1, -0.05, 0.03, 0.7)
+    elif kind == "mesh":
+        lo, hi = (-1.03, -0.61, -0.47), (0.77, 0.52, 0.58)
+        tris = box_mesh(lo, hi)
+        gpu.LBM_SetSolidMesh(g, tris, (*lo, *hi))
+        og.set_solid_mesh(tris, (*lo, *hi))
+    else:
+        gpu.LBM_SetSolidAABB(g, -1.3, -0.6, -0.4, 0.2, 0.5, 0.7)
+        og.set_solid_aabb(-1.3, -0.6, -0.4, 0.2, 0.5, 0.7)
+        gpu.LBM_AddGroundPlane(g, -0.5)
+        og.add_ground_plane(-0.5)
+    assert np.array_equal(gpu.LBM_CopySolidToHost(g), og.solid)
+    assert np.array_equal(gpu.LBM_CopyQToHost(g), og.q)
+    for axis in range(3):
+        assert gpu.LBM_ComputeProjectedArea(g, axis) == og.projected_area(axis)
+
+
+def test_initial_state_is_bit_exact(gpu):
+    g, og = make_pair(48, 24, 24, 0.05, setup_sphere(0, 0, 0, 0.5), u=0.05)
+    f, fn, vel = fields(g)
+    assert np.array_equal(f, og.f) and np.array_equal(fn, og.f_new)
+    assert np.array_equal(vel, og.vel)
+    # force before any step (f = f_new = feq)
+    ex_o, fix_o = og.force()
+    F = gpu.LBM_ComputeDragForceF64(g)
+    assert np.allclose(F, ex_o, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("periodic", [1, 0])
+@pytest.mark.parametrize("op", ["bgk", "mrt", "reg"])
+def test_first_steps_match_oracle(gpu, periodic, op):
+    """Steps 1-3 cover the first collide-only step, both streaming phases and
+    the boundary prepass (Bouzidi q < 1/2 and >= 1/2 on a sphere)."""
+    g, og = make_pair(40, 20, 20, 0.02, setup_sphere(-0.3, 0.1, -0.1, 0.8), tau=0.56,
+                      periodicYZ=periodic, useMRT=int(op == "mrt"), useRegularized=int(op == "reg"),
+                      useSmagorinsky=1, smagorinskyCs=0.1)
+    for n in (1, 1, 1, 5):
+        gpu.LBM_StepN(g, n, 0.05, 0, 0)
+        og.step(0.05, 0, 0, n=n)
+        d = compare(g, og)
+        print(op, periodic, n, d)
+        assert d["f"] < 1e-6 and d["f_new"] < 1e-6 and d["u"] < 1e-6 and d["rho"] < 1e-7
+
+
+def test_config1_bgk_1000_steps(gpu):
+    """Config 1: 128x64x64 BGK, tau 0.6, u=0.05, periodic y/z, 16^3 box,
+    1000 steps; fields, Cd and Cl against the FP32 oracle."""
+    g, og = make_pair(128, 64, 64, 0.1 / 3, setup_aabb(C1_BOX), tau=0.6, periodicYZ=1)
+    A = gpu.LBM_ComputeProjectedArea(g, 0)
+    assert A == 256.0 == og.projected_area(0)
+    gpu.LBM_StepN(g, 1000, 0.05, 0, 0)
+    og.step(0.05, 0, 0, n=1000)
+    d = compare(g, og)
+    print("config1 field differences", d)
+    assert d["u"] < TOL and d["rho"] < TOL and d["f"] < TOL and d["f_new"] < TOL
+    # forces: reference fixed point and FP64 sums
+    ex_o, fix_o = og.force()
+    fx, fy, fz, px, py, pz = gpu.LBM_ComputeDragForceDecomposed(g)
+    dyn = 0.5 * 0.05 ** 2 * A
+    cd, cd_o = abs(fx) / dyn, abs(fix_o[0] / 1e4) / dyn
+    print("Cd fixed-point: gpu %.6f oracle %.6f" % (cd, cd_o))
+    assert abs(cd - cd_o) <= 1e-3 * cd_o
+    assert abs(px - fix_o[4] / 1e4) <= 1e-3 * abs(fix_o[4] / 1e4)
+    F = gpu.LBM_ComputeDragForceF64(g)
+    print("FP64 force gpu", F, "oracle", ex_o)
+    assert abs(F[0] - ex_o[0]) <= 1e-3 * abs(ex_o[0])
+    assert abs(F[3] - ex_o[3]) <= 1e-3 * abs(ex_o[3])
+    assert abs(F[1]) < 1e-3 * abs(F[0]) and abs(F[2]) < 1e-3 * abs(F[0])
+    Cd = gpu.LBM_ComputeDragCoefficient(g, 0.05, A)
+    assert abs(Cd - cd) < 1e-6 * cd
+    assert gpu.LBM_ComputeLiftCoefficient(g, 0.05, A) == pytest.approx(abs(fy) / dyn, rel=1e-6)
+
+
+@pytest.mark.parametrize("op", ["mrt", "reg"])
+def test_smagorinsky_operators_500_steps(gpu, op):
+    g, og = make_pair(64, 32, 32, 0.01, setup_sphere(-1.0, 0.05, -0.02, 0.55), tau=0.53, periodicYZ=0,
+                      useMRT=int(op == "mrt"), useRegularized=int(op == "reg"), useSmagorinsky=1,
+                      smagorinskyCs=0.1, u=0.06)
+    gpu.LBM_StepN(g, 500, 0.06, 0, 0)
+    og.step(0.06, 0, 0, n=500)
+    d = compare(g, og)
+    print(op, d)
+    assert d["u"] < TOL and d["rho"] < TOL and d["f"] < TOL
+
+
+@pytest.mark.parametrize("face", ["inlet", "outlet"])
+def test_ground_plane_body_on_x_faces_and_clamp(gpu, face):
+    """Body cells on an x face (x-edge self copy of lbm_stream.comp:120-122),
+    a ground plane (solid=2 cells collide as fluid), clamped y/z."""
+    def setup(g, og):
+        if face == "inlet":
+            gpu.LBM_SetSolidAABB(g, -4.5, -0.3, -0.2, -3.2, 0.4, 0.5)
+            og.set_solid_aabb(-4.5, -0.3, -0.2, -3.2, 0.4, 0.5)
+        else:
+            gpu.LBM_SetSolidSphere(g, 3.95, 0.1, 0.0, 0.7)
+            og.set_solid_sphere(3.95, 0.1, 0.0, 0.7)
+        gpu.LBM_AddGroundPlane(g, -1.5)
+        og.add_ground_plane(-1.5)
+    g, og = make_pair(48, 24, 24, 0.03, setup, periodicYZ=0, useMRT=1)
+    xs = np.nonzero(og.solid == 1)[0] % 48
+    assert np.any(xs == (0 if face == "inlet" else 47))
+    gpu.LBM_StepN(g, 200, 0.05, 0, 0)
+    og.step(0.05, 0, 0, n=200)
+    d = compare(g, og)
+    print(face, d)
+    assert d["u"] < TOL and d["rho"] < TOL and d["f"] < TOL
+    ex_o, fix_o = og.force()
+    F = gpu.LBM_ComputeDragForceF64(g)
+    assert np.allclose(F, ex_o, rtol=1e-4, atol=1e-6)
+
+
+def test_stability_guard_resets_cells(gpu):
+    """An inlet speed of 0.45 trips the |u| > 0.4 guard at the inlet."""
+    g, og = make_pair(32, 16, 16, 0.05, setup_aabb((-1, -0.5, -0.5, 0, 0.5, 0.5)), periodicYZ=1, u=0.45)
+    gpu.LBM_StepN(g, 20, 0.45, 0, 0)
+    og.step(0.45, 0, 0, n=20)
+    vel = gpu.LBM_CopyVelocityToHost(g)
+    reset = (og.vel[:, 3] == 1) & np.all(og.vel[:, :3] == 0, axis=1) & (og.solid == 0)
+    assert reset.sum() > 0
+    assert np.array_equal(vel[reset], og.vel[reset])
+    d = compare(g, og)
+    print(d)
+    assert d["rho"] < TOL
+
+
+def test_fp64_storage_matches_fp64_oracle(gpu):
+    g, og = make_pair(48, 24, 24, 0.01, setup_sphere(-0.8, 0.0, 0.0, 0.6), tau=0.55, precision=64,
+                      storage=gpu.LBM_STORAGE_FP64, periodicYZ=1, useMRT=1, useSmagorinsky=1)
+    gpu.LBM_StepN(g, 200, 0.05, 0, 0)
+    og.step(0.05, 0, 0, n=200)
+    f, fn = gpu.LBM_CopyPopulationsToHost(g, f64=True)
+    print("fp64", rel_l2(f, og.f), rel_l2(fn, og.f_new))
+    assert rel_l2(f, og.f) < 1e-12 and rel_l2(fn, og.f_new) < 1e-12
+    ex_o, _ = og.force()
+    assert np.allclose(gpu.LBM_ComputeDragForceF64(g), ex_o, rtol=1e-6, atol=1e-9)
+
+
+def test_stepn_equals_repeated_step(gpu):
+    a, _ = make_pair(32, 16, 16, 0.05, setup_sphere(0, 0, 0, 0.6), periodicYZ=0, useMRT=1)
+    b, _ = make_pair(32, 16, 16, 0.05, setup_sphere(0, 0, 0, 0.6), periodicYZ=0, useMRT=1)
+    gpu.LBM_StepN(a, 7, 0.05, 0, 0)
+    for _ in range(7):
+        gpu.LBM_Step(b, 0.05, 0, 0)
+    fa, fna, va = fields(a)
+    fb, fnb, vb = fields(b)
+    assert np.array_equal(fa, fb) and np.array_equal(fna, fnb) and np.array_equal(va, vb)
+    assert a.stepCount == b.stepCount == 7
+
+
+def test_total_mass_fp64(gpu):
+    g, og = make_pair(48, 24, 24, 0.05, setup_sphere(0, 0, 0, 0.5), periodicYZ=1, useMRT=1)
+    gpu.LBM_StepN(g, 50, 0.05, 0, 0)
+    og.step(0.05, 0, 0, n=50)
+    m = gpu.LBM_ComputeTotalMass(g)
+    mo = float(np.sum(og.f, dtype=np.float64))
+    print("mass", m, mo)
+    assert abs(m - mo) < 1e-6 * mo
+
+
+def test_particles_match_oracle(gpu):
+    nx, ny, nz = 64, 32, 32
+    g, og = make_pair(nx, ny, nz, 0.02, setup_sphere(-1.0, 0, 0, 0.6), periodicYZ=1, useMRT=1)
+    gpu.LBM_StepN(g, 100, 0.05, 0, 0)
+    vel = gpu.LBM_CopyVelocityToHost(g)
+    n = 5000
+    parts = gpu.DeviceParticles(n)
+    gpu.LBM_SeedParticles(g, parts, 11)
+    host = O.particles_seed(n, 11)
+    assert np.array_equal(parts.download(), host)
+    host[:50, 0] = 3.98  # these leave the box and respawn
+    parts.upload(host)
+    for step in range(1, 6):
+        gpu.LBM_AdvectParticles(g, parts, 10.0, 8.0 / nx * 4, 11, step)
+        host = O.particles_advect(vel, nx, ny, nz, host, 10.0, 8.0 / nx * 4, 11, step)
+    dev = parts.download()
+    print("particle max diff", np.max(np.abs(dev - host)))
+    assert np.max(np.abs(dev[:, :3] - host[:, :3])) < 1e-5
+    assert np.all(dev[:50, 0] < -3.0)

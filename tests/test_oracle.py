@@ -1,0 +1,459 @@
+"""Pins of the CPU oracle (oracle/) -- runs without a GPU.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign, a transposed index, a wrong relaxation rate) fails:
+the reference's own known-answer tests (simulation/test/test_lbm.c:191-366),
+closed forms (d'Humieres moment polynomials, Zou-He imposed moments, analytic
+sphere intersections, Poiseuille profiles with bounce-back / Bouzidi walls),
+exact invariants (conservation, stream permutation) and brute force on small
+inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+E = O.D3Q19_E
+OPP = O.D3Q19_OPP
+W = O.D3Q19_W
+
+
+# ---------------------------------------------------------------- KATs
+def test_weights_sum_to_one():  # test_lbm.c:191-197
+    assert abs(sum(O.feq(i, 1.0, 0, 0, 0) for i in range(19)) - 1.0) < 1e-6
+
+
+def test_feq_conserves_density():  # test_lbm.c:199-207
+    s = sum(O.feq(i, 1.5, 0.05, 0.01, -0.02) for i in range(19))
+    assert abs(s - 1.5) < 1e-5
+
+
+def test_feq_conserves_momentum():  # test_lbm.c:209-223
+    rho, u = 1.2, np.array([0.08, -0.03, 0.04])
+    f = np.array([O.feq(i, rho, *u) for i in range(19)])
+    assert np.allclose(E.T @ f, rho * u, atol=1e-5)
+
+
+def test_feq_at_rest():  # test_lbm.c:225-232
+    for i in range(19):
+        assert abs(O.feq(i, 1.0, 0, 0, 0) - W[i]) < 1e-6
+
+
+def test_host_feq_matches_shader_feq():
+    for i in range(19):
+        assert O.feq_f32_host(i, 1.0, 0.05, 0, 0) == O.feq(i, 1.0, 0.05, 0, 0)
+
+
+def test_mrt_roundtrip():  # test_lbm.c:300-313
+    f = np.array([O.feq(i, 1.3, 0.06, -0.02, 0.03) for i in range(19)], np.float32)
+    back = O.inverse_mrt(O.forward_mrt(f))
+    assert np.max(np.abs(back - f)) < 1e-5
+
+
+def test_mrt_conserved_moments():  # test_lbm.c:315-328
+    rho, u = 1.4, (0.05, -0.01, 0.02)
+    m = O.forward_mrt(np.array([O.feq(i, rho, *u) for i in range(19)], np.float32))
+    assert abs(m[0] - rho) < 1e-5
+    for k, c in zip((3, 5, 7), u):
+        assert abs(m[k] - rho * c) < 1e-5
+
+
+def test_mrt_equilibrium_fixed_point():  # test_lbm.c:330-366
+    rho, u = 1.1, (0.04, 0.01, -0.02)
+    m = O.forward_mrt(np.array([O.feq(i, rho, *u) for i in range(19)], np.float32))
+    assert np.max(np.abs(m - O.meq(rho, *u))) < 1e-4
+
+
+# ------------------------------------------------- moment basis closed form
+def dhumieres_rows():
+    """The D3Q19 orthogonal basis of d'Humieres et al. (2002) as polynomials of e."""
+    ex, ey, ez = E[:, 0].astype(float), E[:, 1].astype(float), E[:, 2].astype(float)
+    e2 = ex ** 2 + ey ** 2 + ez ** 2
+    return np.array([
+        np.ones(19), 19 * e2 - 30, (21 * e2 ** 2 - 53 * e2 + 24) / 2,
+        ex, (5 * e2 - 9) * ex, ey, (5 * e2 - 9) * ey, ez, (5 * e2 - 9) * ez,
+        3 * ex ** 2 - e2, (3 * e2 - 5) * (3 * ex ** 2 - e2), ey ** 2 - ez ** 2, (3 * e2 - 5) * (ey ** 2 - ez ** 2),
+        ex * ey, ey * ez, ex * ez, (ey ** 2 - ez ** 2) * ex, (ez ** 2 - ex ** 2) * ey, (ex ** 2 - ey ** 2) * ez])
+
+
+def test_forward_mrt_is_dhumieres_basis():
+    M = np.stack([O.forward_mrt(np.eye(19)[j], precision=64) for j in range(19)], axis=1)
+    assert np.array_equal(M, dhumieres_rows())
+
+
+def test_mrt_orthogonality_and_inverse():
+    M = dhumieres_rows()
+    norms = np.array([19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8], float)
+    assert np.allclose(M @ M.T, np.diag(norms))
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        m = rng.normal(size=19)
+        assert np.allclose(O.inverse_mrt(m, precision=64), np.linalg.solve(M, m), atol=1e-13)
+
+
+def test_meq_conserved_and_stress_moments_match_feq():
+    rho, u = 1.05, np.array([0.05, -0.02, 0.03])
+    fe = np.array([O.feq(i, rho, *u, precision=64) for i in range(19)])
+    m = dhumieres_rows() @ fe
+    me = O.meq(rho, *u, precision=64)
+    # conserved moments and the stress moments are exact projections of feq
+    for k in (0, 3, 5, 7, 9, 11, 13, 14, 15):
+        assert abs(m[k] - me[k]) < 1e-14, k
+
+
+# ------------------------------------------------------------ collision
+def near_eq(rng, rho=1.02, u=(0.04, -0.01, 0.02), amp=2e-3):
+    fe = np.array([O.feq(i, rho, *u, precision=64) for i in range(19)])
+    return fe + amp * rng.normal(size=19) * W
+
+
+@pytest.mark.parametrize("op", ["bgk", "mrt", "reg"])
+@pytest.mark.parametrize("smag", [0, 1])
+def test_collision_conserves_mass_and_momentum(op, smag):
+    rng = np.random.default_rng(7)
+    for _ in range(4):
+        f = near_eq(rng)
+        _, fo, vel = O.collide_cell(f, x=3, nx=10, tau=0.55, reg=op == "reg", mrt=op == "mrt", smag=smag,
+                                    precision=64)
+        assert abs(fo.sum() - f.sum()) < 1e-14
+        assert np.allclose(E.T @ fo, E.T @ f, atol=1e-15)
+        assert abs(vel[3] - f.sum()) < 1e-14
+        assert np.allclose(vel[:3], (E.T @ f) / f.sum(), atol=1e-15)
+
+
+@pytest.mark.parametrize("op", ["bgk", "reg"])
+def test_equilibrium_is_fixed_point(op):
+    fe = np.array([O.feq(i, 1.03, 0.05, 0.01, -0.02, precision=64) for i in range(19)])
+    _, fo, _ = O.collide_cell(fe, x=3, nx=10, tau=0.7, reg=op == "reg", precision=64)
+    assert np.max(np.abs(fo - fe)) < 1e-15
+
+
+def stress_neq(f):
+    rho = f.sum()
+    u = (E.T @ f) / rho
+    fe = np.array([O.feq(i, rho, *u, precision=64) for i in range(19)])
+    return np.einsum("ia,ib,i->ab", E, E, f - fe)
+
+
+@pytest.mark.parametrize("op", ["bgk", "mrt", "reg"])
+def test_stress_relaxes_by_one_minus_omega(op):
+    """The traceless non-equilibrium stress decays by (1 - 1/tau) per collision
+    for all three operators (it carries the shear viscosity nu = (tau-1/2)/3)."""
+    rng = np.random.default_rng(3)
+    tau = 0.8
+    f = near_eq(rng)
+    _, fo, _ = O.collide_cell(f, x=3, nx=10, tau=tau, reg=op == "reg", mrt=op == "mrt", precision=64)
+    P, Po = stress_neq(f), stress_neq(fo)
+    dev = lambda A: A - np.trace(A) / 3 * np.eye(3)
+    assert np.allclose(dev(Po), (1 - 1 / tau) * dev(P), atol=1e-12)
+
+
+def test_smagorinsky_tau_eff_solves_quadratic():
+    """tau_eff is the positive root of t^2 - tau t - 4.5 Cs^2 |Pi| / rho = 0
+    (lbm_collide.comp:479-483), read back from the BGK stress decay."""
+    rng = np.random.default_rng(5)
+    tau, cs = 0.52, 0.17
+    f = near_eq(rng, amp=5e-3)
+    _, fo, _ = O.collide_cell(f, x=3, nx=10, tau=tau, smag=1, cs=cs, precision=64)
+    P, Po = stress_neq(f), stress_neq(fo)
+    ratio = Po[0, 1] / P[0, 1]
+    t_eff = 1.0 / (1.0 - ratio)
+    pin = np.sqrt(np.sum(P * P))
+    assert abs(t_eff ** 2 - tau * t_eff - 4.5 * cs ** 2 * pin / f.sum()) < 1e-10
+    assert t_eff > tau
+
+
+def test_body_cell_bounce_back_and_guard():
+    rng = np.random.default_rng(2)
+    f = near_eq(rng)
+    _, fo, vel = O.collide_cell(f, solid=1, precision=64)
+    assert np.array_equal(fo, f[OPP]) and np.array_equal(vel, [0, 0, 0, 1])
+    bad = f.copy()
+    bad[0] = 5.0  # rho > 2 trips the stability guard
+    _, fo, vel = O.collide_cell(bad, precision=64)
+    assert np.allclose(fo, W) and np.array_equal(vel, [0, 0, 0, 1])
+
+
+def test_zou_he_imposes_inlet_velocity_and_outlet_density():
+    rng = np.random.default_rng(4)
+    f = near_eq(rng)
+    fin, _, vel = O.collide_cell(f, x=0, nx=10, inlet=(0.06, 0.0, 0.0), precision=64)
+    assert abs((E.T @ fin)[0] / fin.sum() - 0.06) < 1e-14
+    fout, _, _ = O.collide_cell(f, x=9, nx=10, precision=64)
+    assert abs(fout.sum() - 1.0) < 1e-14
+    # only the unknown directions are rewritten
+    assert np.array_equal(fin[[0, 2, 3, 4, 5, 6, 8, 10, 12, 14, 15, 16, 17, 18]],
+                          f[[0, 2, 3, 4, 5, 6, 8, 10, 12, 14, 15, 16, 17, 18]])
+
+
+# ------------------------------------------------------------ geometry
+C1_BOX = (-2.5, -0.5, -0.5, -1.53125, 0.46875, 0.46875)
+
+
+def brute_links(solid, nx, ny, nz, src_ok, dst_ok):
+    s = solid.reshape(nz, ny, nx)
+    out = np.zeros((nz, ny, nx, 19), bool)
+    for i in range(1, 19):
+        ex, ey, ez = E[i]
+        sh = np.zeros_like(s, dtype=bool)
+        zs = slice(max(0, -ez), nz - max(0, ez))
+        ys = slice(max(0, -ey), ny - max(0, ey))
+        xs = slice(max(0, -ex), nx - max(0, ex))
+        zd = slice(max(0, ez), nz - max(0, -ez) if max(0, -ez) else nz)
+        yd = slice(max(0, ey), ny - max(0, -ey) if max(0, -ey) else ny)
+        xd = slice(max(0, ex), nx - max(0, -ex) if max(0, -ex) else nx)
+        sh[zs, ys, xs] = dst_ok(s[zd, yd, xd])
+        out[..., i] = sh & src_ok(s)
+    return out.reshape(-1, 19)
+
+
+def test_aabb_config1_box_cells_links_and_area():
+    g = O.OracleGrid(128, 64, 64, 0.1 / 3)
+    g.set_solid_aabb(*C1_BOX)
+    s = g.solid.reshape(64, 64, 128)
+    idx = np.argwhere(s == 1)
+    assert idx.min(0).tolist() == [24, 24, 24] and idx.max(0).tolist() == [39, 39, 39]
+    assert len(idx) == 16 ** 3
+    assert g.projected_area(0) == 256 and g.projected_area(1) == 256 and g.projected_area(2) == 256
+    links = brute_links(g.solid, 128, 64, 64, lambda a: a != 1, lambda a: a == 1)
+    q = g.q
+    assert np.array_equal(q >= 0, links)
+    assert np.all(q[links] == 0.5)
+
+
+def test_sphere_q_lies_on_the_surface():
+    nx, ny, nz = 64, 32, 32
+    g = O.OracleGrid(nx, ny, nz, 0.05)
+    cx, cy, cz, r = 0.1, -0.05, 0.03, 0.7
+    g.set_solid_sphere(cx, cy, cz, r)
+    sx, sy, sz = nx / 8, ny / 4, nz / 4
+    cells, dirs = np.nonzero(g.q >= 0)
+    assert len(cells) > 100
+    x = cells % nx
+    y = (cells // nx) % ny
+    z = cells // (nx * ny)
+    q = g.q[cells, dirs].astype(np.float64)
+    p = np.stack([(x + 0.5) / sx - 4 + q * E[dirs, 0] / sx, (y + 0.5) / sy - 2 + q * E[dirs, 1] / sy,
+                  (z + 0.5) / sz - 2 + q * E[dirs, 2] / sz], 1)
+    d = np.linalg.norm(p - [cx, cy, cz], axis=1)
+    inner = (q > 0.0101) & (q < 0.9899)
+    assert inner.sum() > 0.9 * len(q)
+    assert np.max(np.abs(d[inner] - r)) < 2e-5
+    links = brute_links(g.solid, nx, ny, nz, lambda a: a != 1, lambda a: a == 1)
+    assert np.array_equal(g.q >= 0, links)
+
+
+def box_mesh(lo, hi):
+    """12 triangles (12 floats each: v0 xyz pad, v1 xyz pad, v2 xyz pad) of an axis box."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    faces = [(0, 1, 2), (0, 2, 3), (4, 6, 5), (4, 7, 6), (0, 4, 5), (0, 5, 1),
+             (3, 2, 6), (3, 6, 7), (0, 3, 7), (0, 7, 4), (1, 5, 6), (1, 6, 2)]
+    tris = np.zeros((12, 12), np.float32)
+    for t, (a, b, c) in enumerate(faces):
+        tris[t, 0:3], tris[t, 4:7], tris[t, 8:11] = v[a], v[b], v[c]
+    return tris
+
+
+def test_mesh_box_voxelisation_and_link_q():
+    nx, ny, nz = 64, 32, 32
+    lo, hi = (-1.03, -0.61, -0.47), (0.77, 0.52, 0.58)
+    g = O.OracleGrid(nx, ny, nz, 0.05)
+    g.set_solid_mesh(box_mesh(lo, hi), (*lo, *hi))
+    sx, sy, sz = nx / 8, ny / 4, nz / 4
+    X, Y, Z = np.meshgrid((np.arange(nx) + 0.5) / sx - 4, (np.arange(ny) + 0.5) / sy - 2,
+                          (np.arange(nz) + 0.5) / sz - 2, indexing="xy")
+    Xc = ((np.arange(nx) + 0.5) / sx - 4)[None, None, :]
+    Yc = ((np.arange(ny) + 0.5) / sy - 2)[None, :, None]
+    Zc = ((np.arange(nz) + 0.5) / sz - 2)[:, None, None]
+    inside = (Xc > lo[0]) & (Xc < hi[0]) & (Yc > lo[1]) & (Yc < hi[1]) & (Zc > lo[2]) & (Zc < hi[2])
+    assert np.array_equal(g.solid.reshape(nz, ny, nx) == 1, inside)
+    # +x links from the cells in front of the box hit the front face at x = lo[0]
+    cells, dirs = np.nonzero(g.q >= 0)
+    x = cells % nx
+    sel = dirs == 1
+    qexp = (lo[0] - ((x[sel] + 0.5) / sx - 4)) * sx
+    assert np.max(np.abs(g.q[cells[sel], 1] - np.clip(qexp, 0.01, 0.99))) < 1e-4
+    # the reference culls triangles with the +X test for every direction, so
+    # -x links from behind the box find no candidate and fall back to 0.5
+    assert np.all(g.q[cells[dirs == 2], 2] == 0.5)
+
+
+# ------------------------------------------------------------ stream
+def run_box(nx=48, ny=24, nz=24, steps=30, periodic=1, precision=64, op="bgk", tau=0.6):
+    g = O.OracleGrid(nx, ny, nz, 0.05, precision=precision)
+    g.tau = tau
+    g.periodicYZ = periodic
+    g.useMRT = op == "mrt"
+    g.useRegularized = op == "reg"
+    g.set_solid_aabb(-1.0, -0.4, -0.5, 0.2, 0.3, 0.4)
+    g.initialize_flow(0.05, 0, 0)
+    g.step(0.05, 0, 0, n=steps)
+    return g
+
+
+@pytest.mark.parametrize("periodic", [1, 0])
+def test_stream_is_the_pull_permutation_off_boundary(periodic):
+    nx, ny, nz = 48, 24, 24
+    g = run_box(periodic=periodic)
+    f, fn = g.f.reshape(nz, ny, nx, 19), g.f_new.reshape(nz, ny, nx, 19)
+    q = g.q.reshape(nz, ny, nx, 19)
+    for i in range(19):
+        ex, ey, ez = E[i]
+        src = np.roll(fn[..., i], shift=(ez, ey, ex), axis=(0, 1, 2))
+        normal = q[..., OPP[i]] < 0
+        m = normal.copy()
+        if ex == 1:
+            m[:, :, 0] = False
+        if ex == -1:
+            m[:, :, -1] = False
+        if not periodic:
+            if ey:
+                m[:, 0 if ey == 1 else -1, :] = False
+            if ez:
+                m[0 if ez == 1 else -1, :, :] = False
+        assert np.array_equal(f[..., i][m], src[m]), i
+        # x-edge self copy (lbm_stream.comp:120-122)
+        if ex != 0:
+            xe = 0 if ex == 1 else nx - 1
+            sel = normal[:, :, xe]
+            assert np.array_equal(f[:, :, xe, i][sel], fn[:, :, xe, i][sel])
+        if not periodic and ey != 0:
+            ye = 0 if ey == 1 else ny - 1
+            # clamp: the y component of the source stays on the face
+            src_c = np.roll(fn[:, ye, :, i], shift=(ez, ex), axis=(0, 1))
+            mm = normal[:, ye, :].copy()
+            mm[:, 0 if ex == 1 else (nx - 1 if ex == -1 else slice(0, 0))] = False
+            if ez:
+                mm[0 if ez == 1 else -1, :] = False
+            assert np.array_equal(f[:, ye, :, i][mm], src_c[mm]), i
+
+
+def test_halfway_links_reflect_exactly():
+    g = run_box()
+    q, f, fn = g.q, g.f, g.f_new
+    cells, dirs = np.nonzero(q >= 0)
+    # q = 1/2: f_opp(i)(x) = f*_i(x) exactly (inv2q = 1)
+    assert np.array_equal(f[cells, OPP[dirs]], fn[cells, dirs])
+
+
+def channel_profile(qlow, qhigh, steps=6000, tau=0.8, op="bgk"):
+    """Flow through a channel between two solid layers (z = 0 and z = nz-1) with
+    Bouzidi q on the wall links; returns u_x(z) at mid-length."""
+    nx, ny, nz = 96, 2, 20
+    g = O.OracleGrid(nx, ny, nz, 0.05, precision=64)
+    g.tau = tau
+    g.periodicYZ = 1
+    g.useMRT = op == "mrt"
+    solid = np.zeros((nz, ny, nx), np.int32)
+    solid[0] = 1
+    solid[-1] = 1
+    q = -np.ones((nz, ny, nx, 19), np.float32)
+    for i in range(1, 19):
+        if E[i, 2] == -1:
+            q[1, :, :, i] = qlow
+        if E[i, 2] == 1:
+            q[nz - 2, :, :, i] = qhigh
+    g.set_solid_q(solid.reshape(-1), q.reshape(-1, 19))
+    g.initialize_flow(0.02, 0, 0)
+    g.step(0.02, 0, 0, n=steps)
+    v = g.vel.reshape(nz, ny, nx, 4)
+    return v[1:nz - 1, 0, nx // 2, 0]
+
+
+@pytest.mark.parametrize("qlow,qhigh", [(0.5, 0.5), (0.3, 0.7), (0.8, 0.2)])
+def test_channel_wall_position_follows_q(qlow, qhigh):
+    """Poiseuille profile u(z) = a (z - zl)(zh - z): the walls sit q away from the
+    last fluid node (z = 1 - q and z = nz-2 + q), which pins both Bouzidi branches."""
+    u = channel_profile(qlow, qhigh)
+    z = np.arange(1, 19, dtype=float)
+    c = np.polyfit(z, u, 2)
+    roots = np.sort(np.roots(c).real)
+    assert abs(roots[0] - (1 - qlow)) < 0.12, roots
+    assert abs(roots[1] - (18 + qhigh)) < 0.12, roots
+    resid = u - np.polyval(c, z)
+    assert np.max(np.abs(resid)) < 0.02 * np.max(u)
+
+
+# ------------------------------------------------------------ force
+def test_force_matches_brute_force_and_symmetry():
+    g = run_box(nx=48, ny=24, nz=24, steps=200, precision=64)
+    ex_sum, fixed = g.force()
+    f, fn, vel, q, s = g.f, g.f_new, g.vel, g.q, g.solid
+    F = np.zeros(3)
+    P = np.zeros(3)
+    cnt = 0
+    for c in np.nonzero((s == 0) & np.any(q[:, 1:] >= 0, axis=1))[0]:
+        cnt += 1
+        rho, u = vel[c, 3], vel[c, :3]
+        fe = np.array([O.feq(i, rho, *u, precision=64) for i in range(19)])
+        for i in range(1, 19):
+            if q[c, i] >= 0:
+                F += E[i] * (fn[c, i] + f[c, OPP[i]])
+                P += E[i] * (fe[i] + fe[OPP[i]])
+    assert np.allclose(ex_sum[:3], F, rtol=1e-12, atol=1e-12)
+    assert np.allclose(ex_sum[3:], P, rtol=1e-12, atol=1e-12)
+    assert fixed[3] == cnt
+    # fixed point vs exact: each cell rounds once to 1e-4
+    assert np.all(np.abs(fixed[[0, 1, 2]] / 1e4 - F) <= 0.5e-4 * cnt + 1e-9)
+    assert F[0] > 0
+
+
+def test_symmetric_body_has_no_side_force():
+    g = O.OracleGrid(48, 24, 24, 0.05, precision=64)
+    g.tau = 0.7
+    g.periodicYZ = 1
+    g.set_solid_aabb(-1.0, -0.5, -0.5, 0.0, 0.5, 0.5)
+    g.initialize_flow(0.05, 0, 0)
+    g.step(0.05, 0, 0, n=300)
+    ex, _ = g.force()
+    assert ex[0] > 0
+    assert abs(ex[1]) < 1e-9 * abs(ex[0]) and abs(ex[2]) < 1e-9 * abs(ex[0])
+
+
+# ------------------------------------------------------------ samplers / particles
+def test_trilinear_sampler_is_exact_on_linear_fields():
+    nx, ny, nz = 16, 8, 8
+    gx, gy, gz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    vel = np.zeros((nz, ny, nx, 4), np.float32)
+    lin = lambda X, Y, Z: 0.01 + 0.002 * X - 0.003 * Y + 0.004 * Z
+    vel[..., 0] = lin(gx, gy, gz).transpose(2, 1, 0)
+    vel[..., 1] = 0.5 * lin(gx, gy, gz).transpose(2, 1, 0)
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        p = rng.uniform([-3.5, -1.5, -1.5], [3.0, 1.3, 1.3])
+        g = ((p[0] + 4) / 8 * nx, (p[1] + 2) / 4 * ny, (p[2] + 2) / 4 * nz)
+        out = O.sample_trilinear(vel, nx, ny, nz, p)
+        assert abs(out[0] - lin(*g)) < 2e-6
+        assert abs(out[1] - 0.5 * lin(*g)) < 2e-6
+
+
+def test_nearest_sampler_picks_floor_cell():
+    nx, ny, nz = 16, 8, 8
+    vel = np.zeros((nx * ny * nz, 4), np.float32)
+    vel[:, 0] = np.arange(nx * ny * nz)
+    p = (0.3, -0.7, 0.9)
+    g = np.floor([(p[0] + 4) / 8 * nx, (p[1] + 2) / 4 * ny, (p[2] + 2) / 4 * nz]).astype(int)
+    assert O.sample_nearest(vel, nx, ny, nz, p)[0] == g[0] + nx * (g[1] + ny * g[2])
+
+
+def test_rk2_uniform_field_and_respawn():
+    nx, ny, nz = 16, 8, 8
+    vel = np.zeros((nx * ny * nz, 4), np.float32)
+    vel[:, 0] = 0.05
+    vel[:, 2] = -0.01
+    p = O.particles_seed(64, seed=3)
+    assert np.all(p[:, 0] == -4) and np.all(np.abs(p[:, 1:3]) < 2)
+    q = O.particles_advect(vel, nx, ny, nz, p, dt=10.0, vel_scale=0.5, seed=3, step=1)
+    stay = np.abs(p[:, 2] - 10 * 0.5 * 0.01) <= 2.0
+    assert stay.sum() > 50
+    assert np.allclose(q[stay, 0], -4 + 10 * 0.5 * 0.05, atol=1e-6)
+    assert np.allclose(q[stay, 2], p[stay, 2] - 10 * 0.5 * 0.01, atol=1e-6)
+    assert np.all(q[~stay, 0] == -4)
+    far = p.copy()
+    far[:, 0] = 3.99
+    r = O.particles_advect(vel, nx, ny, nz, far, dt=10.0, vel_scale=0.5, seed=3, step=9)
+    assert np.all(r[:, 0] == -4)
+    assert not np.array_equal(r[:, 1], p[:, 1])
