@@ -178,6 +178,29 @@ int LBM_GetBoundaryCellCount(LBMGrid *grid);
 /* Text of the last device or argument error ("" if none). */
 const char *LBM_GetLastError(void);
 
+/* ---- caller-side drag bookkeeping (simulation/src/main.c:1250-1391) ------- */
+
+typedef struct LBMCdMonitor LBMCdMonitor;
+
+/* U: reference velocity; refArea: LBM_ComputeProjectedArea(grid, 0).  The
+ * blockage factor (1 - A/(NY NZ))^2 of main.c:549-552 is taken from the grid.
+ * verbose != 0 prints the reference's "Drag Force: ... Cd=..." lines. */
+LBMCdMonitor *LBM_CdMonitorCreate(LBMGrid *grid, float U, float refArea, int verbose);
+void LBM_CdMonitorFree(LBMCdMonitor *m);
+/* Computes the force and out[6] = {Cd, Cd_corr, Cl (signed), Cd_pressure,
+ * Cd_friction, EMA of Cd_corr}; returns 1 when this sample first meets the
+ * convergence test (relative std of the last 100 Cd samples < 1 %). */
+int LBM_CdMonitorSample(LBMCdMonitor *m, LBMGrid *grid, float out[6]);
+int LBM_CdMonitorConverged(const LBMCdMonitor *m, float *mean, float *relStd);
+/* Strouhal number of the monitor's Cl series (simulation/src/drag_metrics.c:8-68),
+ * one sample per sampleInterval lattice steps. */
+float LBM_CdMonitorStrouhal(const LBMCdMonitor *m, int sampleInterval, float charLength);
+float LBM_ComputeStrouhal(const float *cl, int n, int sampleInterval, float charLength, float U);
+
+/* VTK ImageData dump of velocity, rho and solid to <dir>/field_<step>.vti
+ * (simulation/src/vti_export.c:10-116).  Returns 1 on success. */
+int LBM_WriteVTI(LBMGrid *grid, const char *dir, int step);
+
 #ifdef __cplusplus
 }
 #endif

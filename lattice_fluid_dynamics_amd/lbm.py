@@ -85,6 +85,13 @@ SIGNATURES = {
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
     "LBM_GetLastError": (ctypes.c_char_p, []),
+    "LBM_CdMonitorCreate": (c_void_p, [GridP, c_float, c_float, c_int]),
+    "LBM_CdMonitorFree": (None, [c_void_p]),
+    "LBM_CdMonitorSample": (c_int, [c_void_p, GridP, c_float_p]),
+    "LBM_CdMonitorConverged": (c_int, [c_void_p, c_float_p, c_float_p]),
+    "LBM_CdMonitorStrouhal": (c_float, [c_void_p, c_int, c_float]),
+    "LBM_ComputeStrouhal": (c_float, [c_float_p, c_int, c_int, c_float, c_float]),
+    "LBM_WriteVTI": (c_int, [GridP, ctypes.c_char_p, c_int]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -322,3 +329,45 @@ def LBM_SeedParticles(g: Grid, parts: DeviceParticles, seed: int) -> None:
 
 def LBM_AdvectParticles(g: Grid, parts: DeviceParticles, dt: float, vel_scale: float, seed: int, step: int) -> None:
     load_library().LBM_AdvectParticles(g.ptr, parts.ptr, parts.n, dt, vel_scale, seed, step)
+
+
+class CdMonitor:
+    """The caller-side Cd/Cl bookkeeping of the reference main loop (C, lbm_monitor.c)."""
+
+    def __init__(self, g: Grid, U: float, ref_area: float, verbose: bool = False):
+        self.g = g
+        self.ptr = load_library().LBM_CdMonitorCreate(g.ptr, U, ref_area, int(verbose))
+        if not self.ptr:
+            raise MemoryError("LBM_CdMonitorCreate failed")
+
+    def sample(self):
+        """Returns (newly_converged, dict of Cd, Cd_corr, Cl, Cd_pressure, Cd_friction, ema)."""
+        out = np.zeros(6, np.float32)
+        newly = load_library().LBM_CdMonitorSample(self.ptr, self.g.ptr, _fp(out))
+        keys = ("Cd", "Cd_corr", "Cl", "Cd_pressure", "Cd_friction", "ema")
+        return bool(newly), dict(zip(keys, out.tolist()))
+
+    def converged(self):
+        m, r = c_float(), c_float()
+        ok = load_library().LBM_CdMonitorConverged(self.ptr, ctypes.byref(m), ctypes.byref(r))
+        return bool(ok), m.value, r.value
+
+    def strouhal(self, sample_interval: int, char_length: float) -> float:
+        return float(load_library().LBM_CdMonitorStrouhal(self.ptr, sample_interval, char_length))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                load_library().LBM_CdMonitorFree(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def LBM_ComputeStrouhal(cl: np.ndarray, sample_interval: int, char_length: float, U: float) -> float:
+    c = np.ascontiguousarray(cl, dtype=np.float32)
+    return float(load_library().LBM_ComputeStrouhal(_fp(c), len(c), sample_interval, char_length, U))
+
+
+def LBM_WriteVTI(g: Grid, directory: str, step: int) -> int:
+    return int(load_library().LBM_WriteVTI(g.ptr, directory.encode(), step))
