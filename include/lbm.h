@@ -201,6 +201,53 @@ float LBM_ComputeStrouhal(const float *cl, int n, int sampleInterval, float char
  * (simulation/src/vti_export.c:10-116).  Returns 1 on success. */
 int LBM_WriteVTI(LBMGrid *grid, const char *dir, int step);
 
+/* ---- workgroup-size autotuner (no reference counterpart) ------------------ */
+
+/* Times `steps` steps of the grid for each candidate workgroup size of the
+ * step kernel (128, 256, 512 lanes) with the given inlet velocity, restores
+ * the simulation state after each trial, keeps the fastest and returns it
+ * (0 on failure).  msPerStep (3 floats, may be NULL) receives the timings. */
+int LBM_Autotune(LBMGrid *grid, int steps, float inletVelX, float inletVelY, float inletVelZ,
+                 float *msPerStep);
+
+/* Attainable HBM bandwidth in GB/s: read + write bytes of a plain device copy
+ * of `bytes` bytes over its mean launch time (iters launches). */
+float LBM_MeasureCopyBandwidth(unsigned long long bytes, int iters);
+
+/* ---- batched parameter sweeps (no reference counterpart) ------------------ */
+
+typedef struct LBMBatch LBMBatch;
+
+/* All tunnels of a batch share the collision operator, Smagorinsky constant,
+ * Y/Z boundary mode and population storage format. */
+LBMBatch *LBM_BatchCreate(int populationStorage, int useRegularized, int useMRT, int useSmagorinsky,
+                          float smagorinskyCs, int periodicYZ);
+void LBM_BatchFree(LBMBatch *batch);
+/* Copies the solid mask and Bouzidi links set on `geometry` (LBM_SetSolid*,
+ * LBM_AddGroundPlane); the grid may be freed afterwards.  Each tunnel has its
+ * own size, tau and inlet velocity.  Returns the tunnel index (-1 on error,
+ * or after LBM_BatchInitialize). */
+int LBM_BatchAddTunnel(LBMBatch *batch, LBMGrid *geometry, float tau, float inletX, float inletY,
+                       float inletZ);
+/* Allocates every tunnel in one device buffer and sets f = feq(1, u_inlet). */
+int LBM_BatchInitialize(LBMBatch *batch);
+/* n steps of every tunnel: one launch per step over all tunnels (plus the
+ * boundary prepass), replayed from HIP graphs two steps at a time.  Each
+ * tunnel evolves exactly as a single grid with the same settings would. */
+void LBM_BatchStep(LBMBatch *batch, int n);
+int LBM_BatchProfile(LBMBatch *batch, int n, int useGraph, float *ms);
+int LBM_BatchAutotune(LBMBatch *batch, int steps, float *msPerStep);
+int LBM_BatchSetBlockSize(LBMBatch *batch, int blockSize);
+int LBM_BatchTunnelCount(const LBMBatch *batch);
+long long LBM_BatchTotalCells(const LBMBatch *batch);
+/* out: 6 doubles per tunnel (F xyz, pressure part xyz, FP64 sums); fixedOut
+ * (may be NULL): the 7 reference fixed-point accumulators per tunnel. */
+int LBM_BatchComputeForcesF64(LBMBatch *batch, double *out, long long *fixedOut);
+/* out: 4 floats per tunnel: Cd, Cl (signed), Cd_pressure, Cd_friction. */
+int LBM_BatchComputeCoefficients(LBMBatch *batch, float *out);
+int LBM_BatchCopyPopulationsToHostF64(LBMBatch *batch, int tunnel, double *f, double *fNew);
+int LBM_BatchCopyVelocityToHost(LBMBatch *batch, int tunnel, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -573,21 +573,46 @@ void LBM_SetSolidMesh(LBMGrid *g, float *triangles, int numTriangles, float minX
 
 /* ------------------------------------------------------------ device sync */
 
+/* Whether cell c belongs to the boundary list: it has Bouzidi links, or it
+ * lies on a clamped Y/Z face, or it is a body cell on an x face. */
+static int in_boundary_list(const LBMGrid *g, const Impl *im, long c, int periodic) {
+    int x = (int)(c % g->sizeX);
+    long t = c / g->sizeX;
+    int y = (int)(t % g->sizeY), z = (int)(t / g->sizeY);
+    int face = !periodic && (y == 0 || y == g->sizeY - 1 || z == 0 || z == g->sizeZ - 1);
+    int xbody = im->solid[c] == 1 && (x == 0 || x == g->sizeX - 1);
+    return im->qRow[c] >= 0 || face || xbody;
+}
+
+int lbm_boundary_count(const LBMGrid *g, int periodic) {
+    const Impl *im = (const Impl *)g->impl;
+    int n = 0;
+    for (long c = 0; c < g->totalCells; c++) n += in_boundary_list(g, im, c, periodic);
+    return n;
+}
+
+/* Writes the list in cell order: cells[k0 + j] and q[i*qStride + k0 + j]. */
+void lbm_boundary_fill(const LBMGrid *g, int periodic, int32_t *cells, float *q, long qStride, int k0) {
+    const Impl *im = (const Impl *)g->impl;
+    int k = k0;
+    for (long c = 0; c < g->totalCells; c++) {
+        if (!in_boundary_list(g, im, c, periodic)) continue;
+        cells[k] = (int32_t)c;
+        for (int i = 0; i < 19; i++) q[(long)i * qStride + k] = q_get((Impl *)im, c, i);
+        k++;
+    }
+}
+
+const int32_t *lbm_host_solid(const LBMGrid *g) { return ((const Impl *)g->impl)->solid; }
+
+LbmDev *lbm_device_of(LBMGrid *g) { return impl_of(g)->dev; }
+
 /* Builds the boundary list for the current periodicYZ and uploads geometry. */
 static int ensure_geometry(LBMGrid *g) {
     Impl *im = impl_of(g);
     const int periodic = g->periodicYZ == 1;
     if (!im->geomDirty && im->builtPeriodic == periodic) return 1;
-    const long nc = g->totalCells;
-    int n = 0;
-    for (long c = 0; c < nc; c++) {
-        int x = (int)(c % g->sizeX);
-        long t = c / g->sizeX;
-        int y = (int)(t % g->sizeY), z = (int)(t / g->sizeY);
-        int face = !periodic && (y == 0 || y == g->sizeY - 1 || z == 0 || z == g->sizeZ - 1);
-        int xbody = im->solid[c] == 1 && (x == 0 || x == g->sizeX - 1);
-        if (im->qRow[c] >= 0 || face || xbody) n++;
-    }
+    const int n = lbm_boundary_count(g, periodic);
     int32_t *cells = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     float *q = (float *)malloc(sizeof(float) * 19 * (size_t)(n > 0 ? n : 1));
     if (!cells || !q) {
@@ -596,18 +621,7 @@ static int ensure_geometry(LBMGrid *g) {
         note_error("boundary list: host allocation failed");
         return 0;
     }
-    int k = 0;
-    for (long c = 0; c < nc; c++) {
-        int x = (int)(c % g->sizeX);
-        long t = c / g->sizeX;
-        int y = (int)(t % g->sizeY), z = (int)(t / g->sizeY);
-        int face = !periodic && (y == 0 || y == g->sizeY - 1 || z == 0 || z == g->sizeZ - 1);
-        int xbody = im->solid[c] == 1 && (x == 0 || x == g->sizeX - 1);
-        if (!(im->qRow[c] >= 0 || face || xbody)) continue;
-        cells[k] = (int32_t)c;
-        for (int i = 0; i < 19; i++) q[(long)i * n + k] = q_get(im, c, i);
-        k++;
-    }
+    lbm_boundary_fill(g, periodic, cells, q, n, 0);
     int ok = lbmdev_set_geometry(im->dev, im->solid, n, cells, q);
     free(cells);
     free(q);
@@ -846,4 +860,27 @@ int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes) {
 }
 int LBM_CopyToHost(void *dst, const void *src, unsigned long long bytes) {
     return lbmdev_memcpy_d2h(dst, src, (size_t)bytes);
+}
+
+/* Workgroup-size autotuner for the step kernel: times `steps` steps with each
+ * candidate (128, 256, 512 lanes) on the grid's own lattice and keeps the
+ * fastest.  The populations and velocity are saved first and restored after
+ * every trial, so tuning leaves the simulation state unchanged. */
+int LBM_Autotune(LBMGrid *g, int steps, float ux, float uy, float uz, float *msPerStep) {
+    if (!g || steps <= 0) return 0;
+    Impl *im = impl_of(g);
+    if (!im->flowReady || !ensure_geometry(g)) return 0;
+    LbmStepParams p;
+    step_params(g, ux, uy, uz, &p);
+    int best = lbmdev_autotune(im->dev, &p, steps, msPerStep);
+    if (!best) note_error(lbmdev_last_error());
+    return best;
+}
+
+/* Attainable HBM bandwidth: a plain device copy of `bytes` bytes, read +
+ * write bytes over the mean launch time. */
+float LBM_MeasureCopyBandwidth(unsigned long long bytes, int iters) {
+    float gbps = 0.0f;
+    if (!lbmdev_copy_bandwidth((size_t)bytes, iters > 0 ? iters : 10, &gbps, NULL)) note_error(lbmdev_last_error());
+    return gbps;
 }

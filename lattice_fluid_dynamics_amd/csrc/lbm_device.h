@@ -75,6 +75,43 @@ int lbmdev_synchronize(LbmDev *d);
 int lbmdev_profile_steps(LbmDev *d, const LbmStepParams *p, int nsteps, float *total_ms,
                          float *main_ms, float *pre_ms);
 
+int lbmdev_set_block_size(LbmDev *d, int bs);
+/* Times `steps` steps per candidate workgroup size (128, 256, 512), restoring
+ * the populations and velocity after each trial; sets and returns the best. */
+int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, float *ms_per_step);
+int lbmdev_block_size(const LbmDev *d);
+int lbmdev_export_populations_f64(LbmDev *d, int periodicYZ, double *f_post_stream, double *f_post_collision);
+void *lbmdev_stream(LbmDev *d);
+
+/* ---- batched tunnels: one allocation, one launch per step over all ---- */
+typedef struct LbmBatchDev LbmBatchDev;
+typedef struct {
+    float smagorinskyCs;
+    int useRegularized, useMRT, useSmagorinsky, periodicYZ;
+} LbmBatchParams;
+
+/* dims: 3 ints per tunnel; tau: 1 per tunnel; inlet: 3 per tunnel. */
+LbmBatchDev *lbmbatch_create(int storage, int ntunnels, const int *dims, const float *tau, const float *inlet,
+                             char *err, int errlen);
+void lbmbatch_destroy(LbmBatchDev *b);
+/* solid: concatenated per tunnel; n_ovr_per: list length per tunnel; ovr_cells:
+ * concatenated local cell indices; ovr_q: q[i*Ntot + k] over the whole list. */
+int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const int *n_ovr_per, const int32_t *ovr_cells,
+                          const float *ovr_q);
+int lbmbatch_init_flow(LbmBatchDev *b, const float *feq, const float *vel4);
+int lbmbatch_step(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph);
+int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph, float *ms);
+int lbmbatch_set_block_size(LbmBatchDev *b, int bs);
+int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int steps, float *ms_per_step);
+int lbmbatch_block_size(const LbmBatchDev *b);
+int lbmbatch_force(LbmBatchDev *b, int periodic, double *exact, long long *fixed);
+int lbmbatch_export(LbmBatchDev *b, int t, int periodic, double *f_post_stream, double *f_post_collision);
+int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host);
+int lbmbatch_synchronize(LbmBatchDev *b);
+
+/* Read + write bandwidth of a plain copy of `bytes` bytes (mean of iters). */
+int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float *ms_per_copy);
+
 void *lbmdev_malloc(size_t bytes);
 void lbmdev_free(void *p);
 int lbmdev_memcpy_h2d(void *dst, const void *src, size_t bytes);

@@ -71,13 +71,13 @@ template <> struct Pop<LBMDEV_FP64> {
     static __device__ __forceinline__ S enc(T v, int) { return v; }
 };
 template <> struct Pop<LBMDEV_FP16S> {
-    typedef __half S;
+    typedef uint16_t S;  // IEEE binary16 bits
     typedef float T;
     static __device__ __forceinline__ T dec(S v, int i) {
-        return fmaf(__half2float(v), 1.0f / 32768.0f, wgt<float>(i));
+        return fmaf(__half2float(__ushort_as_half(v)), 1.0f / 32768.0f, wgt<float>(i));
     }
     static __device__ __forceinline__ S enc(T v, int i) {
-        return __float2half_rn((v - wgt<float>(i)) * 32768.0f);
+        return __half_as_ushort(__float2half_rn((v - wgt<float>(i)) * 32768.0f));
     }
 };
 template <> struct Pop<LBMDEV_BF16S> {
@@ -96,7 +96,39 @@ template <> struct Pop<LBMDEV_BF16S> {
 struct Geo {
     int nx, ny, nz, ncell;
     long long stride;
+    // n / nx and n / ny for 0 <= n < 2^31 as umulhi(n, m) >> s (m = 0: divisor 1);
+    // Granlund & Montgomery's round-up method with a 31-bit numerator
+    uint32_t mx, my;
+    int sx, sy;
 };
+
+static void fast_div_magic(uint32_t d, uint32_t &m, int &s) {
+    if (d <= 1) {
+        m = 0;
+        s = 0;
+        return;
+    }
+    int l = 0;
+    while ((1ull << l) < d) l++;  // ceil(log2 d)
+    m = (uint32_t)((1ull << (31 + l)) / d + 1);
+    s = l - 1;
+}
+
+static Geo make_geo(int nx, int ny, int nz, long long stride) {
+    Geo g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.ncell = nx * ny * nz;
+    g.stride = stride;
+    fast_div_magic((uint32_t)nx, g.mx, g.sx);
+    fast_div_magic((uint32_t)ny, g.my, g.sy);
+    return g;
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, int s) {
+    return m == 0 ? n : (__umulhi(n, m) >> s);
+}
 
 // Each of the 8 XCDs receives workgroups b, b+8, b+16, ... in dispatch order;
 // this gives every XCD one contiguous range of cells so neighbouring rows and
@@ -107,10 +139,26 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 }
 
 __device__ __forceinline__ void decompose(const Geo &g, int cell, int &x, int &y, int &z) {
-    x = cell % g.nx;
-    int t = cell / g.nx;
-    y = t % g.ny;
-    z = t / g.ny;
+    const uint32_t t = fast_div((uint32_t)cell, g.mx, g.sx);
+    x = cell - (int)t * g.nx;
+    const uint32_t zz = fast_div(t, g.my, g.sy);
+    y = (int)t - (int)zz * g.ny;
+    z = (int)zz;
+}
+
+// Base of direction plane i: uniform across the workgroup, so loads take it
+// in scalar registers and only a 32-bit per-lane offset.
+template <typename S>
+__device__ __forceinline__ S *plane(S *pop, const Geo &g, int i) {
+    return pop + (long long)i * g.stride;
+}
+
+// Element idx of a plane, addressed as uniform base + 32-bit byte offset
+// (the plane is far below 4 GiB), which maps onto the scalar-base form of
+// the global load/store instructions.
+template <typename S>
+__device__ __forceinline__ S &at(S *base, uint32_t idx) {
+    return *(S *)((char *)base + idx * (uint32_t)sizeof(S));
 }
 
 __device__ __forceinline__ int cidx(const Geo &g, int x, int y, int z) {
@@ -198,6 +246,16 @@ __device__ __forceinline__ float tsqrt<float>(float v) { return sqrtf(v); }
 template <>
 __device__ __forceinline__ double tsqrt<double>(double v) { return sqrt(v); }
 
+// |u| > 0.4 of the guard.  In FP32, sqrtf(u2) > 0.4f holds exactly when
+// u2 > 0.1600000113... (the largest float whose correctly rounded square root
+// is <= 0.4f is 0x3E23D70B), so the square root is not needed.
+template <typename T>
+__device__ __forceinline__ bool speed_exceeds(T u2);
+template <>
+__device__ __forceinline__ bool speed_exceeds<float>(float u2) { return u2 > __uint_as_float(0x3E23D70Bu); }
+template <>
+__device__ __forceinline__ bool speed_exceeds<double>(double u2) { return sqrt(u2) > 0.40; }
+
 struct CollideParams {
     float tau, cs2x18, inx, iny, inz;
     int smag;
@@ -249,8 +307,8 @@ __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollidePa
         uz = jz;
     }
     // stability guard (lbm_collide.comp:429-440): reset to feq(1, 0) = w
-    T umag = tsqrt<T>(ux * ux + uy * uy + uz * uz);
-    if (rho < T(0.3) || rho > T(2.0) || umag > T(0.40) || isinf(rho) || isnan(rho)) {
+    if (rho < T(0.3) || rho > T(2.0) || speed_exceeds<T>(ux * ux + uy * uy + uz * uz) || isinf(rho) ||
+        isnan(rho)) {
 #pragma unroll
         for (int i = 0; i < 19; i++) fo[i] = wgt<T>(i);
         rho = T(1);
@@ -400,14 +458,13 @@ struct StepArgs {
     CollideParams cp;
 };
 
-// The fused stream + collide kernel.  PHASE 0 = A, 1 = B (see file header).
+// One cell of the fused stream + collide update.  PHASE 0 = A, 1 = B (see
+// file header).
 template <int FMT, int OP, int PHASE>
-__global__ __launch_bounds__(BLOCK) void k_step(StepArgs<FMT> a) {
+__device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell) {
     typedef Pop<FMT> P;
     typedef typename P::T T;
     const Geo &g = a.g;
-    const int cell = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (cell >= g.ncell) return;
     int x, y, z;
     decompose(g, cell, x, y, z);
     // torus neighbour offsets
@@ -425,13 +482,13 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs<FMT> a) {
         for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
     } else if (PHASE == 0) {
 #pragma unroll
-        for (int i = 0; i < 19; i++) f[i] = P::dec(a.pop[(long long)i * g.stride + cell], i);
+        for (int i = 0; i < 19; i++) f[i] = P::dec(at(plane(a.pop, g, i), (uint32_t)cell), i);
     } else {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const int j = OPP[i];
-            const int src = X[1 - EX[i]] + Y[1 - EY[i]] + Z[1 - EZ[i]];
-            f[i] = P::dec(a.pop[(long long)j * g.stride + src], j);
+            const uint32_t src = (uint32_t)(X[1 - EX[i]] + Y[1 - EY[i]] + Z[1 - EZ[i]]);
+            f[i] = P::dec(at(plane(a.pop, g, j), src), j);
         }
     }
 
@@ -451,15 +508,77 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs<FMT> a) {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const int j = OPP[i];
-            a.pop[(long long)j * g.stride + cell] = P::enc(fo[i], j);
+            at(plane(a.pop, g, j), (uint32_t)cell) = P::enc(fo[i], j);
         }
     } else {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
-            const int dst = X[1 + EX[i]] + Y[1 + EY[i]] + Z[1 + EZ[i]];
-            a.pop[(long long)i * g.stride + dst] = P::enc(fo[i], i);
+            const uint32_t dst = (uint32_t)(X[1 + EX[i]] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
+            at(plane(a.pop, g, i), dst) = P::enc(fo[i], i);
         }
     }
+}
+
+// The fused kernel over one lattice, BS lanes per workgroup.
+template <int FMT, int OP, int PHASE, int BS>
+__global__ __launch_bounds__(BS) void k_step(StepArgs<FMT> a) {
+    const int cell = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (cell >= a.g.ncell) return;
+    step_cell<FMT, OP, PHASE>(a, cell);
+}
+
+// ---------------------------------------------------------- batched tunnels
+// Many lattices of different sizes and parameters advanced by one launch:
+// workgroup b (after the XCD remap) belongs to tunnel block_tunnel[b] and
+// covers cells (b - blockStart) * BS + [0, BS) of it.  Every cell runs the
+// same step_cell as the single-lattice kernel.
+struct TunnelDesc {
+    Geo g;
+    long long popOff;  // element offset of the tunnel's 19 planes
+    int cellOff;       // offset into the concatenated per-cell arrays
+    int blockStart;    // first workgroup of the tunnel
+    float tau, inx, iny, inz;
+};
+
+template <int FMT>
+struct BatchArgs {
+    typename Pop<FMT>::S *pop;
+    const uint8_t *mask;
+    const int32_t *ovr_index;
+    const typename Pop<FMT>::T *ovr_vals;
+    int ovr_stride;
+    int use_ovr;
+    float4 *vel;
+    int write_vel;
+    float cs2x18;
+    int smag;
+    const TunnelDesc *desc;
+    const int32_t *block_tunnel;
+};
+
+template <int FMT, int OP, int PHASE, int BS>
+__global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const TunnelDesc d = b.desc[b.block_tunnel[blk]];
+    const int cell = (blk - d.blockStart) * BS + threadIdx.x;
+    if (cell >= d.g.ncell) return;
+    StepArgs<FMT> a;
+    a.pop = b.pop + d.popOff;
+    a.mask = b.mask + d.cellOff;
+    a.ovr_index = b.ovr_index + d.cellOff;
+    a.ovr_vals = b.ovr_vals;
+    a.ovr_stride = b.ovr_stride;
+    a.use_ovr = b.use_ovr;
+    a.vel = b.vel + d.cellOff;
+    a.write_vel = b.write_vel;
+    a.g = d.g;
+    a.cp.tau = d.tau;
+    a.cp.cs2x18 = b.cs2x18;
+    a.cp.inx = d.inx;
+    a.cp.iny = d.iny;
+    a.cp.inz = d.inz;
+    a.cp.smag = b.smag;
+    step_cell<FMT, OP, PHASE>(a, cell);
 }
 
 template <int FMT>
@@ -486,6 +605,23 @@ __global__ __launch_bounds__(BLOCK) void k_prepass(BoundArgs<FMT> a) {
         const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
         a.ovr_vals[(long long)i * a.n_ovr + k] =
             stream_value<FMT, STATE>(a.pop, a.g, a.periodic, x, y, z, i, qo);
+    }
+}
+
+// Batched boundary prepass: list entry k belongs to tunnel ovr_tunnel[k].
+template <int FMT, int STATE>
+__global__ __launch_bounds__(BLOCK) void k_prepass_batch(BoundArgs<FMT> a, const int32_t *ovr_tunnel,
+                                                         const TunnelDesc *desc) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= a.n_ovr) return;
+    const TunnelDesc d = desc[ovr_tunnel[k]];
+    const typename Pop<FMT>::S *pop = a.pop + d.popOff;
+    int x, y, z;
+    decompose(d.g, a.ovr_cells[k], x, y, z);
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
+        a.ovr_vals[(long long)i * a.n_ovr + k] = stream_value<FMT, STATE>(pop, d.g, a.periodic, x, y, z, i, qo);
     }
 }
 
@@ -539,6 +675,7 @@ struct ForceArgs {
     const float *ovr_q;
     const float4 *vel;
     int n_ovr;
+    int q_stride;  // plane stride of ovr_q (n_ovr for one lattice, the batch total otherwise)
     int periodic;
     Geo g;
     double *pd;
@@ -564,7 +701,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(ForceArgs<FMT> a) {
         if (a.mask[cell] & (LBMDEV_MASK_BODY | LBMDEV_MASK_GROUND)) continue;
         float q[19];
 #pragma unroll
-        for (int i = 1; i < 19; i++) q[i] = a.ovr_q[(long long)i * a.n_ovr + k];
+        for (int i = 1; i < 19; i++) q[i] = a.ovr_q[(long long)i * a.q_stride + k];
         bool any = false;
 #pragma unroll
         for (int i = 1; i < 19; i++) any |= (q[i] >= 0.0f);
@@ -603,19 +740,34 @@ __global__ __launch_bounds__(BLOCK) void k_force(ForceArgs<FMT> a) {
     block_reduce_store<6, 7>(acc, fix, a.pd, a.pi);
 }
 
-// Sums per-block partials in block order (deterministic).
+// Sums per-block partials in a fixed order (deterministic): lane l adds
+// blocks l, l+64, ... then lane 0 adds the 64 lane sums in lane order.
 template <int NV, int NI>
 __global__ __launch_bounds__(64) void k_finalize(const double *pd, const long long *pi, int nblocks,
                                                  double *outd, long long *outi) {
-    if (threadIdx.x != 0) return;
+    __shared__ double sd[64][NV > 0 ? NV : 1];
+    __shared__ long long si[64][NI > 0 ? NI : 1];
+    const int l = threadIdx.x;
     for (int k = 0; k < NV; k++) {
         double s = 0;
-        for (int b = 0; b < nblocks; b++) s += pd[(long long)b * NV + k];
+        for (int b = l; b < nblocks; b += 64) s += pd[(long long)b * NV + k];
+        sd[l][k] = s;
+    }
+    for (int k = 0; k < NI; k++) {
+        long long s = 0;
+        for (int b = l; b < nblocks; b += 64) s += pi[(long long)b * NI + k];
+        si[l][k] = s;
+    }
+    __syncthreads();
+    if (l != 0) return;
+    for (int k = 0; k < NV; k++) {
+        double s = 0;
+        for (int j = 0; j < 64; j++) s += sd[j][k];
         outd[k] = s;
     }
     for (int k = 0; k < NI; k++) {
         long long s = 0;
-        for (int b = 0; b < nblocks; b++) s += pi[(long long)b * NI + k];
+        for (int j = 0; j < 64; j++) s += si[j][k];
         outi[k] = s;
     }
 }
@@ -821,6 +973,7 @@ struct LbmDev {
     long long *resi;
     float *feq_d;
     int nparts;  // partial slots allocated
+    int bs;      // workgroup size of the step kernel
 };
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
@@ -851,6 +1004,7 @@ extern "C" LbmDev *lbmdev_create(int nx, int ny, int nz, int storage, char *err,
     d->telem = storage == LBMDEV_FP64 ? 8 : 4;
     d->stride = (nc + 63) / 64 * 64;
     d->state = -1;
+    d->bs = 256;
     hipError_t e;
     const size_t popBytes = (size_t)19 * d->stride * d->elem;
     d->nparts = (int)((nc + BLOCK - 1) / BLOCK);
@@ -999,12 +1153,34 @@ static void launch_prepass(LbmDev *d, int periodic) {
     a.ovr_vals = (typename Pop<FMT>::T *)d->ovr_vals;
     a.n_ovr = d->n_ovr;
     a.periodic = periodic;
-    a.g = Geo{d->nx, d->ny, d->nz, d->ncell, d->stride};
+    a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
     const int nb = (d->n_ovr + BLOCK - 1) / BLOCK;
     if (d->state == ST_A)
         hipLaunchKernelGGL((k_prepass<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
     else
         hipLaunchKernelGGL((k_prepass<FMT, ST_B>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
+}
+
+template <int FMT, int OP, int BS>
+static void launch_step_phase(int phase, int ncell, hipStream_t st, const StepArgs<FMT> &a) {
+    const int nb = (ncell + BS - 1) / BS;
+    if (phase == 0)
+        hipLaunchKernelGGL((k_step<FMT, OP, 0, BS>), dim3(nb), dim3(BS), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_step<FMT, OP, 1, BS>), dim3(nb), dim3(BS), 0, st, a);
+}
+
+// Workgroup sizes the autotuner chooses from.
+static const int kBlockSizes[3] = {128, 256, 512};
+
+template <int FMT, int OP>
+static void launch_step_bs(int bs, int phase, int ncell, hipStream_t st, const StepArgs<FMT> &a) {
+    if (bs == 128)
+        launch_step_phase<FMT, OP, 128>(phase, ncell, st, a);
+    else if (bs == 512)
+        launch_step_phase<FMT, OP, 512>(phase, ncell, st, a);
+    else
+        launch_step_phase<FMT, OP, 256>(phase, ncell, st, a);
 }
 
 template <int FMT, int OP>
@@ -1018,18 +1194,14 @@ static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ov
     a.use_ovr = use_ovr && d->n_ovr > 0;
     a.vel = d->vel;
     a.write_vel = write_vel;
-    a.g = Geo{d->nx, d->ny, d->nz, d->ncell, d->stride};
+    a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
     a.cp.tau = p->tau;
     a.cp.cs2x18 = 18.0f * p->smagorinskyCs * p->smagorinskyCs;
     a.cp.inx = p->inletX;
     a.cp.iny = p->inletY;
     a.cp.inz = p->inletZ;
     a.cp.smag = p->useSmagorinsky == 1;
-    const int nb = (d->ncell + BLOCK - 1) / BLOCK;
-    if (phase == 0)
-        hipLaunchKernelGGL((k_step<FMT, OP, 0>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
-    else
-        hipLaunchKernelGGL((k_step<FMT, OP, 1>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
+    launch_step_bs<FMT, OP>(d->bs, phase, d->ncell, d->stream, a);
 }
 
 template <int FMT>
@@ -1140,8 +1312,9 @@ static void launch_force(LbmDev *d, int periodic, int nb) {
     a.ovr_q = d->ovr_q;
     a.vel = d->vel;
     a.n_ovr = d->n_ovr;
+    a.q_stride = d->n_ovr;
     a.periodic = periodic;
-    a.g = Geo{d->nx, d->ny, d->nz, d->ncell, d->stride};
+    a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
     a.pd = d->pd;
     a.pi = d->pi;
     hipLaunchKernelGGL((k_force<FMT, STATE>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
@@ -1189,7 +1362,7 @@ static void launch_export(LbmDev *d, int periodic, void *os, void *oc, int as_do
     a.ovr_q = d->ovr_q;
     a.n_ovr = d->n_ovr;
     a.periodic = periodic;
-    a.g = Geo{d->nx, d->ny, d->nz, d->ncell, d->stride};
+    a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
     a.out_stream = os;
     a.out_collision = oc;
     a.as_double = as_double;
@@ -1322,4 +1495,684 @@ extern "C" int lbmdev_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+// ======================================================================
+// Batched tunnels (C ABI): many lattices in one allocation, one launch per
+// step covering all of them, the step sequence replayed from HIP graphs.
+// ======================================================================
+
+struct LbmBatchDev {
+    int fmt;
+    size_t elem, telem;
+    int nt;
+    TunnelDesc *desc_h;
+    TunnelDesc *desc_d;
+    int *dims;  // 3 per tunnel
+    long long total_cells, total_pop;
+    void *pop;
+    uint8_t *mask;
+    int32_t *ovr_index;
+    float4 *vel;
+    int n_ovr;
+    int *kstart;  // nt + 1 list offsets per tunnel (host)
+    int32_t *ovr_cells, *ovr_tunnel;
+    float *ovr_q;
+    void *ovr_vals;
+    int32_t *block_tunnel;
+    int nblocks, bs;
+    int state;
+    hipStream_t stream;
+    double *pd;
+    long long *pi;
+    double *resd;
+    long long *resi;
+    int nparts;
+    hipGraphExec_t graph[2];
+    int graph_ok;
+    LbmBatchParams graph_params;
+    int graph_bs;
+};
+
+extern "C" void lbmbatch_destroy(LbmBatchDev *b);
+
+static void batch_free_graphs(LbmBatchDev *b) {
+    for (int k = 0; k < 2; k++)
+        if (b->graph[k]) {
+            hipGraphExecDestroy(b->graph[k]);
+            b->graph[k] = NULL;
+        }
+    b->graph_ok = 0;
+}
+
+// Lays the workgroups of every tunnel end to end for workgroup size bs.
+static int batch_layout_blocks(LbmBatchDev *b, int bs) {
+    int nb = 0;
+    for (int t = 0; t < b->nt; t++) nb += (b->desc_h[t].g.ncell + bs - 1) / bs;
+    int32_t *map = (int32_t *)malloc(sizeof(int32_t) * (size_t)nb);
+    if (!map) return 0;
+    int at = 0;
+    for (int t = 0; t < b->nt; t++) {
+        b->desc_h[t].blockStart = at;
+        const int n = (b->desc_h[t].g.ncell + bs - 1) / bs;
+        for (int k = 0; k < n; k++) map[at + k] = t;
+        at += n;
+    }
+    hipFree(b->block_tunnel);
+    b->block_tunnel = NULL;
+    hipError_t e = hipMalloc((void **)&b->block_tunnel, sizeof(int32_t) * (size_t)nb);
+    if (e == hipSuccess) e = hipMemcpy(b->block_tunnel, map, sizeof(int32_t) * (size_t)nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->desc_d, b->desc_h, sizeof(TunnelDesc) * (size_t)b->nt, hipMemcpyHostToDevice);
+    free(map);
+    if (e != hipSuccess) return set_err("batch layout", e);
+    b->nblocks = nb;
+    b->bs = bs;
+    batch_free_graphs(b);
+    return 1;
+}
+
+extern "C" LbmBatchDev *lbmbatch_create(int storage, int nt, const int *dims, const float *tau,
+                                        const float *inlet, char *err, int errlen) {
+    LbmBatchDev *b = (LbmBatchDev *)calloc(1, sizeof(LbmBatchDev));
+    if (!b) return NULL;
+    b->fmt = storage;
+    b->elem = elem_size(storage);
+    b->telem = storage == LBMDEV_FP64 ? 8 : 4;
+    b->nt = nt;
+    b->state = -1;
+    b->desc_h = (TunnelDesc *)calloc((size_t)nt, sizeof(TunnelDesc));
+    b->dims = (int *)malloc(sizeof(int) * 3 * (size_t)nt);
+    b->kstart = (int *)calloc((size_t)nt + 1, sizeof(int));
+    if (!b->desc_h || !b->dims || !b->kstart) {
+        snprintf(err, errlen, "host allocation failed");
+        lbmbatch_destroy(b);
+        return NULL;
+    }
+    long long cells = 0, popel = 0;
+    for (int t = 0; t < nt; t++) {
+        const int nx = dims[3 * t], ny = dims[3 * t + 1], nz = dims[3 * t + 2];
+        const long long nc = (long long)nx * ny * nz;
+        if (nx < 2 || ny < 1 || nz < 1 || cells + nc > 0x7fffffffLL - 256) {
+            snprintf(err, errlen, "tunnel %d (%dx%dx%d) is outside the supported range", t, nx, ny, nz);
+            lbmbatch_destroy(b);
+            return NULL;
+        }
+        memcpy(b->dims + 3 * t, dims + 3 * t, 3 * sizeof(int));
+        TunnelDesc &d = b->desc_h[t];
+        d.g = make_geo(nx, ny, nz, (nc + 63) / 64 * 64);
+        d.popOff = popel;
+        d.cellOff = (int)cells;
+        d.tau = tau[t];
+        d.inx = inlet[3 * t];
+        d.iny = inlet[3 * t + 1];
+        d.inz = inlet[3 * t + 2];
+        cells += nc;
+        popel += 19 * d.g.stride;
+    }
+    b->total_cells = cells;
+    b->total_pop = popel;
+    b->nparts = (int)((cells + BLOCK - 1) / BLOCK);
+    if (b->nparts < FORCE_BLOCKS) b->nparts = FORCE_BLOCKS;
+    hipError_t e;
+#define BALLOC(ptr, bytes, name)                                                          \
+    if ((e = hipMalloc((void **)&(ptr), (bytes))) != hipSuccess) {                        \
+        snprintf(err, errlen, "device allocation of %s (%.1f MB) failed: %s", name,     \
+                 (double)(bytes) / 1048576.0, hipGetErrorString(e));                      \
+        lbmbatch_destroy(b);                                                              \
+        return NULL;                                                                      \
+    }
+    BALLOC(b->pop, (size_t)popel * b->elem, "batched populations");
+    BALLOC(b->mask, (size_t)cells, "mask");
+    BALLOC(b->ovr_index, (size_t)cells * 4, "boundary index");
+    BALLOC(b->vel, (size_t)cells * 16, "velocity");
+    BALLOC(b->desc_d, sizeof(TunnelDesc) * (size_t)nt, "tunnel table");
+    BALLOC(b->pd, (size_t)b->nparts * 8 * 8, "reduction partials");
+    BALLOC(b->pi, (size_t)b->nparts * 8 * 8, "reduction partials");
+    BALLOC(b->resd, (size_t)nt * 8 * 8, "reduction results");
+    BALLOC(b->resi, (size_t)nt * 8 * 8, "reduction results");
+#undef BALLOC
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
+        snprintf(err, errlen, "stream creation failed: %s", hipGetErrorString(e));
+        lbmbatch_destroy(b);
+        return NULL;
+    }
+    if (!batch_layout_blocks(b, 256)) {
+        snprintf(err, errlen, "%s", g_err);
+        lbmbatch_destroy(b);
+        return NULL;
+    }
+    return b;
+}
+
+extern "C" void lbmbatch_destroy(LbmBatchDev *b) {
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_free_graphs(b);
+    hipFree(b->pop);
+    hipFree(b->mask);
+    hipFree(b->ovr_index);
+    hipFree(b->vel);
+    hipFree(b->desc_d);
+    hipFree(b->ovr_cells);
+    hipFree(b->ovr_tunnel);
+    hipFree(b->ovr_q);
+    hipFree(b->ovr_vals);
+    hipFree(b->block_tunnel);
+    hipFree(b->pd);
+    hipFree(b->pi);
+    hipFree(b->resd);
+    hipFree(b->resi);
+    if (b->stream) hipStreamDestroy(b->stream);
+    free(b->desc_h);
+    free(b->dims);
+    free(b->kstart);
+    free(b);
+}
+
+// solid: concatenated per tunnel; n_ovr_per[t] list entries of tunnel t;
+// ovr_cells: concatenated local cell indices; ovr_q: planes q[i*Ntot + k].
+extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const int *n_ovr_per,
+                                     const int32_t *ovr_cells, const float *ovr_q) {
+    HIPCHECK(hipStreamSynchronize(b->stream), "synchronize");
+    const long long nc = b->total_cells;
+    int ntot = 0;
+    for (int t = 0; t < b->nt; t++) {
+        b->kstart[t] = ntot;
+        ntot += n_ovr_per[t];
+    }
+    b->kstart[b->nt] = ntot;
+    uint8_t *mask = (uint8_t *)malloc((size_t)nc);
+    int32_t *index = (int32_t *)malloc((size_t)nc * 4);
+    int32_t *tun = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ntot > 0 ? ntot : 1));
+    if (!mask || !index || !tun) {
+        free(mask);
+        free(index);
+        free(tun);
+        snprintf(g_err, sizeof(g_err), "host allocation failed");
+        return 0;
+    }
+    for (long long c = 0; c < nc; c++) {
+        mask[c] = (solid[c] == 1 ? LBMDEV_MASK_BODY : 0) | (solid[c] == 2 ? LBMDEV_MASK_GROUND : 0);
+        index[c] = -1;
+    }
+    for (int t = 0; t < b->nt; t++)
+        for (int k = b->kstart[t]; k < b->kstart[t + 1]; k++) {
+            const long long c = b->desc_h[t].cellOff + (long long)ovr_cells[k];
+            mask[c] |= LBMDEV_MASK_OVR;
+            index[c] = k;
+            tun[k] = t;
+        }
+    hipFree(b->ovr_cells);
+    hipFree(b->ovr_tunnel);
+    hipFree(b->ovr_q);
+    hipFree(b->ovr_vals);
+    b->ovr_cells = b->ovr_tunnel = NULL;
+    b->ovr_q = NULL;
+    b->ovr_vals = NULL;
+    hipError_t e = hipSuccess;
+    const size_t n1 = (size_t)(ntot > 0 ? ntot : 1);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_cells, n1 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_tunnel, n1 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_q, n1 * 19 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_vals, n1 * 19 * b->telem);
+    if (e == hipSuccess) e = hipMemcpy(b->mask, mask, (size_t)nc, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->ovr_index, index, (size_t)nc * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_cells, ovr_cells, (size_t)ntot * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_tunnel, tun, (size_t)ntot * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ntot > 0)
+        e = hipMemcpy(b->ovr_q, ovr_q, (size_t)ntot * 19 * 4, hipMemcpyHostToDevice);
+    free(mask);
+    free(index);
+    free(tun);
+    if (e != hipSuccess) return set_err("batch geometry upload", e);
+    b->n_ovr = ntot;
+    batch_free_graphs(b);
+    return 1;
+}
+
+template <int FMT>
+static void batch_init_t(LbmBatchDev *b, const float *feq_d, const float *vel4) {
+    typedef typename Pop<FMT>::S S;
+    for (int t = 0; t < b->nt; t++) {
+        const TunnelDesc &d = b->desc_h[t];
+        const int nb = (d.g.ncell + BLOCK - 1) / BLOCK;
+        float4 v4 = make_float4(vel4[4 * t], vel4[4 * t + 1], vel4[4 * t + 2], vel4[4 * t + 3]);
+        hipLaunchKernelGGL(k_init<FMT>, dim3(nb), dim3(BLOCK), 0, b->stream, (S *)b->pop + d.popOff, d.g,
+                           b->vel + d.cellOff, feq_d + 19 * t, v4);
+    }
+}
+
+extern "C" int lbmbatch_init_flow(LbmBatchDev *b, const float *feq, const float *vel4) {
+    float *feq_d = NULL;
+    HIPCHECK(hipMalloc((void **)&feq_d, sizeof(float) * 19 * (size_t)b->nt), "feq table");
+    hipError_t e = hipMemcpy(feq_d, feq, sizeof(float) * 19 * (size_t)b->nt, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        switch (b->fmt) {
+        case LBMDEV_FP32: batch_init_t<LBMDEV_FP32>(b, feq_d, vel4); break;
+        case LBMDEV_FP16S: batch_init_t<LBMDEV_FP16S>(b, feq_d, vel4); break;
+        case LBMDEV_BF16S: batch_init_t<LBMDEV_BF16S>(b, feq_d, vel4); break;
+        case LBMDEV_FP64: batch_init_t<LBMDEV_FP64>(b, feq_d, vel4); break;
+        }
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(b->stream);
+    hipFree(feq_d);
+    if (e != hipSuccess) return set_err("batch init", e);
+    if (e2 != hipSuccess) return set_err("batch init", e2);
+    b->state = ST_INIT;
+    return 1;
+}
+
+template <int FMT, int OP, int BS>
+static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase, int use_ovr, int wv) {
+    BatchArgs<FMT> a;
+    a.pop = (typename Pop<FMT>::S *)b->pop;
+    a.mask = b->mask;
+    a.ovr_index = b->ovr_index;
+    a.ovr_vals = (const typename Pop<FMT>::T *)b->ovr_vals;
+    a.ovr_stride = b->n_ovr;
+    a.use_ovr = use_ovr && b->n_ovr > 0;
+    a.vel = b->vel;
+    a.write_vel = wv;
+    a.cs2x18 = 18.0f * p->smagorinskyCs * p->smagorinskyCs;
+    a.smag = p->useSmagorinsky == 1;
+    a.desc = b->desc_d;
+    a.block_tunnel = b->block_tunnel;
+    if (phase == 0)
+        hipLaunchKernelGGL((k_step_batch<FMT, OP, 0, BS>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+    else
+        hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+}
+
+template <int FMT, int OP>
+static void batch_launch_main_bs(LbmBatchDev *b, const LbmBatchParams *p, int phase, int use_ovr, int wv) {
+    if (b->bs == 128)
+        batch_launch_main<FMT, OP, 128>(b, p, phase, use_ovr, wv);
+    else if (b->bs == 512)
+        batch_launch_main<FMT, OP, 512>(b, p, phase, use_ovr, wv);
+    else
+        batch_launch_main<FMT, OP, 256>(b, p, phase, use_ovr, wv);
+}
+
+template <int FMT>
+static void batch_step_once(LbmBatchDev *b, const LbmBatchParams *p, int wv) {
+    int phase, use_ovr;
+    if (b->state == ST_INIT) {
+        phase = 0;
+        use_ovr = 0;
+    } else {
+        if (b->n_ovr > 0) {
+            BoundArgs<FMT> a;
+            a.pop = (const typename Pop<FMT>::S *)b->pop;
+            a.ovr_cells = b->ovr_cells;
+            a.ovr_q = b->ovr_q;
+            a.ovr_vals = (typename Pop<FMT>::T *)b->ovr_vals;
+            a.n_ovr = b->n_ovr;
+            a.periodic = p->periodicYZ == 1;
+            a.g = make_geo(2, 1, 1, 64);
+            const int nb = (b->n_ovr + BLOCK - 1) / BLOCK;
+            if (b->state == ST_A)
+                hipLaunchKernelGGL((k_prepass_batch<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, b->stream, a,
+                                   (const int32_t *)b->ovr_tunnel, (const TunnelDesc *)b->desc_d);
+            else
+                hipLaunchKernelGGL((k_prepass_batch<FMT, ST_B>), dim3(nb), dim3(BLOCK), 0, b->stream, a,
+                                   (const int32_t *)b->ovr_tunnel, (const TunnelDesc *)b->desc_d);
+        }
+        phase = b->state == ST_A ? 1 : 0;
+        use_ovr = 1;
+    }
+    if (p->useRegularized == 1)
+        batch_launch_main_bs<FMT, OP_REG>(b, p, phase, use_ovr, wv);
+    else if (p->useMRT == 1)
+        batch_launch_main_bs<FMT, OP_MRT>(b, p, phase, use_ovr, wv);
+    else
+        batch_launch_main_bs<FMT, OP_BGK>(b, p, phase, use_ovr, wv);
+    b->state = phase == 0 ? ST_A : ST_B;
+}
+
+static void batch_step_any(LbmBatchDev *b, const LbmBatchParams *p, int wv) {
+    switch (b->fmt) {
+    case LBMDEV_FP32: batch_step_once<LBMDEV_FP32>(b, p, wv); break;
+    case LBMDEV_FP16S: batch_step_once<LBMDEV_FP16S>(b, p, wv); break;
+    case LBMDEV_BF16S: batch_step_once<LBMDEV_BF16S>(b, p, wv); break;
+    case LBMDEV_FP64: batch_step_once<LBMDEV_FP64>(b, p, wv); break;
+    }
+}
+
+static int same_params(const LbmBatchParams *a, const LbmBatchParams *c) {
+    return a->smagorinskyCs == c->smagorinskyCs && a->useRegularized == c->useRegularized &&
+           a->useMRT == c->useMRT && a->useSmagorinsky == c->useSmagorinsky && a->periodicYZ == c->periodicYZ;
+}
+
+// Captures the two-step sequence (phase B then phase A, each with its
+// prepass) that returns the stored layout to phase-A order; graph[1] writes
+// the velocity buffer on its second step, graph[0] does not.
+static int batch_capture(LbmBatchDev *b, const LbmBatchParams *p) {
+    batch_free_graphs(b);
+    for (int k = 0; k < 2; k++) {
+        hipGraph_t g = NULL;
+        HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+        const int saved = b->state;
+        batch_step_any(b, p, 0);
+        batch_step_any(b, p, k);
+        b->state = saved;
+        HIPCHECK(hipStreamEndCapture(b->stream, &g), "graph capture");
+        hipError_t e = hipGraphInstantiate(&b->graph[k], g, NULL, NULL, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err("graph instantiate", e);
+    }
+    b->graph_ok = 1;
+    b->graph_params = *p;
+    b->graph_bs = b->bs;
+    return 1;
+}
+
+extern "C" int lbmbatch_step(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph) {
+    if (b->state < 0) {
+        snprintf(g_err, sizeof(g_err), "batch stepped before initialisation");
+        return 0;
+    }
+    while (nsteps > 0 && b->state != ST_A) {  // bring the layout to phase-A order
+        batch_step_any(b, p, nsteps == 1);
+        nsteps--;
+    }
+    if (use_graph && nsteps >= 2) {
+        if (!b->graph_ok || !same_params(&b->graph_params, p) || b->graph_bs != b->bs)
+            if (!batch_capture(b, p)) return 0;
+        const int pairs = nsteps / 2, rem = nsteps % 2;
+        for (int k = 0; k < pairs; k++) {
+            const int last = (k == pairs - 1) && rem == 0;
+            HIPCHECK(hipGraphLaunch(b->graph[last ? 1 : 0], b->stream), "graph launch");
+        }
+        nsteps = rem;  // state is phase-A order again
+    }
+    for (int s = 0; s < nsteps; s++) batch_step_any(b, p, s == nsteps - 1);
+    HIPCHECK(hipGetLastError(), "batch step launch");
+    return 1;
+}
+
+extern "C" int lbmbatch_set_block_size(LbmBatchDev *b, int bs) {
+    if (bs != 128 && bs != 256 && bs != 512) return 0;
+    if (bs == b->bs) return 1;
+    HIPCHECK(hipStreamSynchronize(b->stream), "synchronize");
+    return batch_layout_blocks(b, bs);
+}
+
+extern "C" int lbmbatch_block_size(const LbmBatchDev *b) { return b->bs; }
+
+extern "C" int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph, float *ms) {
+    hipEvent_t t0, t1;
+    HIPCHECK(hipEventCreate(&t0), "event");
+    HIPCHECK(hipEventCreate(&t1), "event");
+    HIPCHECK(hipStreamSynchronize(b->stream), "synchronize");
+    hipEventRecord(t0, b->stream);
+    int ok = lbmbatch_step(b, p, nsteps, use_graph);
+    hipEventRecord(t1, b->stream);
+    hipError_t e = hipEventSynchronize(t1);
+    float v = 0;
+    hipEventElapsedTime(&v, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    if (e != hipSuccess) return set_err("batch profile", e);
+    *ms = v;
+    return ok;
+}
+
+template <int FMT, int STATE>
+static void batch_force_t(LbmBatchDev *b, int periodic) {
+    for (int t = 0; t < b->nt; t++) {
+        const int k0 = b->kstart[t], n = b->kstart[t + 1] - k0;
+        const TunnelDesc &d = b->desc_h[t];
+        int nb = (n + BLOCK - 1) / BLOCK;
+        if (nb > FORCE_BLOCKS) nb = FORCE_BLOCKS;
+        if (nb < 1) nb = 1;
+        ForceArgs<FMT> a;
+        a.pop = (const typename Pop<FMT>::S *)b->pop + d.popOff;
+        a.mask = b->mask + d.cellOff;
+        a.ovr_cells = b->ovr_cells + k0;
+        a.ovr_q = b->ovr_q + k0;
+        a.vel = b->vel + d.cellOff;
+        a.n_ovr = n;
+        a.q_stride = b->n_ovr;
+        a.periodic = periodic;
+        a.g = d.g;
+        a.pd = b->pd;
+        a.pi = b->pi;
+        hipLaunchKernelGGL((k_force<FMT, STATE>), dim3(nb), dim3(BLOCK), 0, b->stream, a);
+        hipLaunchKernelGGL((k_finalize<6, 7>), dim3(1), dim3(64), 0, b->stream, b->pd, b->pi, nb, b->resd + 6 * t,
+                           b->resi + 7 * t);
+    }
+}
+
+template <int FMT>
+static void batch_force_state(LbmBatchDev *b, int periodic) {
+    if (b->state == ST_INIT) batch_force_t<FMT, ST_INIT>(b, periodic);
+    else if (b->state == ST_A) batch_force_t<FMT, ST_A>(b, periodic);
+    else batch_force_t<FMT, ST_B>(b, periodic);
+}
+
+extern "C" int lbmbatch_force(LbmBatchDev *b, int periodic, double *exact, long long *fixed) {
+    if (b->state < 0) return 0;
+    switch (b->fmt) {
+    case LBMDEV_FP32: batch_force_state<LBMDEV_FP32>(b, periodic); break;
+    case LBMDEV_FP16S: batch_force_state<LBMDEV_FP16S>(b, periodic); break;
+    case LBMDEV_BF16S: batch_force_state<LBMDEV_BF16S>(b, periodic); break;
+    case LBMDEV_FP64: batch_force_state<LBMDEV_FP64>(b, periodic); break;
+    }
+    HIPCHECK(hipGetLastError(), "batch force launch");
+    HIPCHECK(hipMemcpyAsync(exact, b->resd, sizeof(double) * 6 * (size_t)b->nt, hipMemcpyDeviceToHost, b->stream),
+             "batch force readback");
+    HIPCHECK(hipMemcpyAsync(fixed, b->resi, sizeof(long long) * 7 * (size_t)b->nt, hipMemcpyDeviceToHost,
+                            b->stream),
+             "batch force readback");
+    HIPCHECK(hipStreamSynchronize(b->stream), "batch force");
+    return 1;
+}
+
+template <int FMT>
+static void batch_export_t(LbmBatchDev *b, int t, int periodic, void *os, void *oc) {
+    const TunnelDesc &d = b->desc_h[t];
+    ExportArgs<FMT> a;
+    a.pop = (const typename Pop<FMT>::S *)b->pop + d.popOff;
+    a.mask = b->mask + d.cellOff;
+    a.ovr_index = b->ovr_index + d.cellOff;
+    a.ovr_q = b->ovr_q;
+    a.n_ovr = b->n_ovr;
+    a.periodic = periodic;
+    a.g = d.g;
+    a.out_stream = os;
+    a.out_collision = oc;
+    a.as_double = 1;
+    a.pd = NULL;
+    a.pi = NULL;
+    const int nb = (d.g.ncell + BLOCK - 1) / BLOCK;
+    if (b->state == ST_INIT)
+        hipLaunchKernelGGL((k_export<FMT, ST_INIT>), dim3(nb), dim3(BLOCK), 0, b->stream, a);
+    else if (b->state == ST_A)
+        hipLaunchKernelGGL((k_export<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, b->stream, a);
+    else
+        hipLaunchKernelGGL((k_export<FMT, ST_B>), dim3(nb), dim3(BLOCK), 0, b->stream, a);
+}
+
+extern "C" int lbmbatch_export(LbmBatchDev *b, int t, int periodic, double *f_ps, double *f_pc) {
+    if (b->state < 0 || t < 0 || t >= b->nt) return 0;
+    const size_t bytes = (size_t)b->desc_h[t].g.ncell * 19 * 8;
+    void *dps = NULL, *dpc = NULL;
+    HIPCHECK(hipMalloc(&dps, bytes), "export buffer");
+    hipError_t e = hipMalloc(&dpc, bytes);
+    if (e != hipSuccess) {
+        hipFree(dps);
+        return set_err("export buffer", e);
+    }
+    switch (b->fmt) {
+    case LBMDEV_FP32: batch_export_t<LBMDEV_FP32>(b, t, periodic, dps, dpc); break;
+    case LBMDEV_FP16S: batch_export_t<LBMDEV_FP16S>(b, t, periodic, dps, dpc); break;
+    case LBMDEV_BF16S: batch_export_t<LBMDEV_BF16S>(b, t, periodic, dps, dpc); break;
+    case LBMDEV_FP64: batch_export_t<LBMDEV_FP64>(b, t, periodic, dps, dpc); break;
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess && f_ps) e = hipMemcpyAsync(f_ps, dps, bytes, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess && f_pc) e = hipMemcpyAsync(f_pc, dpc, bytes, hipMemcpyDeviceToHost, b->stream);
+    hipError_t e2 = hipStreamSynchronize(b->stream);
+    hipFree(dps);
+    hipFree(dpc);
+    if (e != hipSuccess) return set_err("batch export", e);
+    if (e2 != hipSuccess) return set_err("batch export", e2);
+    return 1;
+}
+
+extern "C" int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host) {
+    if (t < 0 || t >= b->nt) return 0;
+    const TunnelDesc &d = b->desc_h[t];
+    HIPCHECK(hipMemcpyAsync(host, b->vel + d.cellOff, (size_t)d.g.ncell * 16, hipMemcpyDeviceToHost, b->stream),
+             "velocity readback");
+    HIPCHECK(hipStreamSynchronize(b->stream), "velocity readback");
+    return 1;
+}
+
+extern "C" int lbmbatch_synchronize(LbmBatchDev *b) {
+    HIPCHECK(hipStreamSynchronize(b->stream), "synchronize");
+    return 1;
+}
+
+// Single-lattice workgroup size (autotuned by the host).
+extern "C" int lbmdev_set_block_size(LbmDev *d, int bs) {
+    if (bs != 128 && bs != 256 && bs != 512) return 0;
+    d->bs = bs;
+    return 1;
+}
+
+extern "C" int lbmdev_block_size(const LbmDev *d) { return d->bs; }
+
+// ---------------------------------------------------------------- autotuner
+// Candidates are timed on the real lattice; the stored state (populations,
+// velocity, layout) is copied aside first and put back after every trial.
+struct Snapshot {
+    void *pop;
+    void *vel;
+    size_t popBytes, velBytes;
+};
+
+static int snapshot_take(Snapshot &s, const void *pop, size_t pb, const void *vel, size_t vb, hipStream_t st) {
+    s.popBytes = pb;
+    s.velBytes = vb;
+    s.pop = s.vel = NULL;
+    HIPCHECK(hipMalloc(&s.pop, pb), "autotune snapshot");
+    hipError_t e = hipMalloc(&s.vel, vb);
+    if (e != hipSuccess) {
+        hipFree(s.pop);
+        return set_err("autotune snapshot", e);
+    }
+    HIPCHECK(hipMemcpyAsync(s.pop, pop, pb, hipMemcpyDeviceToDevice, st), "autotune snapshot");
+    HIPCHECK(hipMemcpyAsync(s.vel, vel, vb, hipMemcpyDeviceToDevice, st), "autotune snapshot");
+    return 1;
+}
+
+static int snapshot_put(const Snapshot &s, void *pop, void *vel, hipStream_t st) {
+    HIPCHECK(hipMemcpyAsync(pop, s.pop, s.popBytes, hipMemcpyDeviceToDevice, st), "autotune restore");
+    HIPCHECK(hipMemcpyAsync(vel, s.vel, s.velBytes, hipMemcpyDeviceToDevice, st), "autotune restore");
+    HIPCHECK(hipStreamSynchronize(st), "autotune restore");
+    return 1;
+}
+
+extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, float *ms_per_step) {
+    if (d->state < 0 || steps <= 0) return 0;
+    Snapshot s;
+    const size_t pb = (size_t)19 * d->stride * d->elem, vb = (size_t)d->ncell * 16;
+    if (!snapshot_take(s, d->pop, pb, d->vel, vb, d->stream)) return 0;
+    const int state0 = d->state, bs0 = d->bs;
+    int best = 0;
+    float bestMs = 0;
+    int ok = 1;
+    for (int c = 0; c < 3 && ok; c++) {
+        d->bs = kBlockSizes[c];
+        float tot = 0;
+        ok = dispatch_steps(d, p, 2, NULL) &&  // warm-up
+             lbmdev_profile_steps(d, p, steps, &tot, NULL, NULL);
+        const float ms = tot / steps;
+        if (ms_per_step) ms_per_step[c] = ms;
+        if (ok && (best == 0 || ms < bestMs)) {
+            best = kBlockSizes[c];
+            bestMs = ms;
+        }
+        ok = ok && snapshot_put(s, d->pop, d->vel, d->stream);
+        d->state = state0;
+    }
+    hipFree(s.pop);
+    hipFree(s.vel);
+    d->bs = ok ? best : bs0;
+    return ok ? best : 0;
+}
+
+extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int steps, float *ms_per_step) {
+    if (b->state < 0 || steps <= 0) return 0;
+    Snapshot s;
+    const size_t pb = (size_t)b->total_pop * b->elem, vb = (size_t)b->total_cells * 16;
+    if (!snapshot_take(s, b->pop, pb, b->vel, vb, b->stream)) return 0;
+    const int state0 = b->state, bs0 = b->bs;
+    int best = 0;
+    float bestMs = 0;
+    int ok = 1;
+    for (int c = 0; c < 3 && ok; c++) {
+        ok = lbmbatch_set_block_size(b, kBlockSizes[c]);
+        float ms = 0;
+        ok = ok && lbmbatch_step(b, p, 2, 1) && lbmbatch_profile(b, p, steps, 1, &ms);
+        ms /= steps;
+        if (ms_per_step) ms_per_step[c] = ms;
+        if (ok && (best == 0 || ms < bestMs)) {
+            best = kBlockSizes[c];
+            bestMs = ms;
+        }
+        ok = ok && snapshot_put(s, b->pop, b->vel, b->stream);
+        b->state = state0;
+    }
+    hipFree(s.pop);
+    hipFree(s.vel);
+    if (!lbmbatch_set_block_size(b, ok ? best : bs0)) return 0;
+    return ok ? best : 0;
+}
+
+// ------------------------------------------------------ copy bandwidth probe
+// A plain streaming copy (16 B per lane per iteration, grid-stride) used to
+// measure the attainable HBM bandwidth and to calibrate byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+extern "C" int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float *ms_per_copy) {
+    const long long n = (long long)(bytes / 16);
+    float4 *a = NULL, *b = NULL;
+    HIPCHECK(hipMalloc((void **)&a, (size_t)n * 16), "copy probe buffer");
+    hipError_t e = hipMalloc((void **)&b, (size_t)n * 16);
+    if (e != hipSuccess) {
+        hipFree(a);
+        return set_err("copy probe buffer", e);
+    }
+    hipMemset(a, 0, (size_t)n * 16);
+    hipMemset(b, 0, (size_t)n * 16);
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = cus * 16;
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);  // warm-up
+    hipEventRecord(t0, 0);
+    for (int k = 0; k < iters; k++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+    hipEventRecord(t1, 0);
+    e = hipEventSynchronize(t1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return set_err("copy probe", e);
+    const float per = ms / iters;
+    if (ms_per_copy) *ms_per_copy = per;
+    if (gbps) *gbps = (float)(2.0 * (double)n * 16 / (per * 1e-3) / 1e9);
+    return 1;
 }

@@ -92,6 +92,22 @@ SIGNATURES = {
     "LBM_CdMonitorStrouhal": (c_float, [c_void_p, c_int, c_float]),
     "LBM_ComputeStrouhal": (c_float, [c_float_p, c_int, c_int, c_float, c_float]),
     "LBM_WriteVTI": (c_int, [GridP, ctypes.c_char_p, c_int]),
+    "LBM_Autotune": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p]),
+    "LBM_MeasureCopyBandwidth": (c_float, [ctypes.c_ulonglong, c_int]),
+    "LBM_BatchCreate": (c_void_p, [c_int, c_int, c_int, c_int, c_float, c_int]),
+    "LBM_BatchFree": (None, [c_void_p]),
+    "LBM_BatchAddTunnel": (c_int, [c_void_p, GridP, c_float, c_float, c_float, c_float]),
+    "LBM_BatchInitialize": (c_int, [c_void_p]),
+    "LBM_BatchStep": (None, [c_void_p, c_int]),
+    "LBM_BatchProfile": (c_int, [c_void_p, c_int, c_int, c_float_p]),
+    "LBM_BatchAutotune": (c_int, [c_void_p, c_int, c_float_p]),
+    "LBM_BatchSetBlockSize": (c_int, [c_void_p, c_int]),
+    "LBM_BatchTunnelCount": (c_int, [c_void_p]),
+    "LBM_BatchTotalCells": (ctypes.c_longlong, [c_void_p]),
+    "LBM_BatchComputeForcesF64": (c_int, [c_void_p, c_double_p, ctypes.POINTER(ctypes.c_longlong)]),
+    "LBM_BatchComputeCoefficients": (c_int, [c_void_p, c_float_p]),
+    "LBM_BatchCopyPopulationsToHostF64": (c_int, [c_void_p, c_int, c_double_p, c_double_p]),
+    "LBM_BatchCopyVelocityToHost": (c_int, [c_void_p, c_int, c_float_p]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -371,3 +387,102 @@ def LBM_ComputeStrouhal(cl: np.ndarray, sample_interval: int, char_length: float
 
 def LBM_WriteVTI(g: Grid, directory: str, step: int) -> int:
     return int(load_library().LBM_WriteVTI(g.ptr, directory.encode(), step))
+
+
+def LBM_Autotune(g: Grid, steps: int, ux: float, uy: float = 0.0, uz: float = 0.0):
+    """Returns (chosen workgroup size, ms per step for 128/256/512 lanes)."""
+    ms = np.zeros(3, np.float32)
+    bs = load_library().LBM_Autotune(g.ptr, steps, ux, uy, uz, _fp(ms))
+    if not bs:
+        raise RuntimeError(last_error())
+    return int(bs), ms
+
+
+class Batch:
+    """Many tunnels advanced together (include/lbm.h LBM_Batch*)."""
+
+    def __init__(self, storage=LBM_STORAGE_FP32, useRegularized=0, useMRT=1, useSmagorinsky=0,
+                 smagorinskyCs=0.1, periodicYZ=1):
+        self.ptr = load_library().LBM_BatchCreate(storage, useRegularized, useMRT, useSmagorinsky,
+                                                   smagorinskyCs, periodicYZ)
+        if not self.ptr:
+            raise MemoryError("LBM_BatchCreate failed")
+
+    def add(self, geometry: Grid, tau: float, ux: float, uy: float = 0.0, uz: float = 0.0) -> int:
+        k = load_library().LBM_BatchAddTunnel(self.ptr, geometry.ptr, tau, ux, uy, uz)
+        if k < 0:
+            raise RuntimeError("LBM_BatchAddTunnel failed")
+        return k
+
+    def initialize(self):
+        if not load_library().LBM_BatchInitialize(self.ptr):
+            raise RuntimeError("LBM_BatchInitialize failed: " + last_error())
+
+    def step(self, n: int):
+        load_library().LBM_BatchStep(self.ptr, n)
+
+    def profile(self, n: int, use_graph: bool = True) -> float:
+        ms = c_float()
+        if not load_library().LBM_BatchProfile(self.ptr, n, int(use_graph), ctypes.byref(ms)):
+            raise RuntimeError(last_error())
+        return ms.value
+
+    def autotune(self, steps: int):
+        ms = np.zeros(3, np.float32)
+        bs = load_library().LBM_BatchAutotune(self.ptr, steps, _fp(ms))
+        if not bs:
+            raise RuntimeError(last_error())
+        return int(bs), ms
+
+    def set_block_size(self, bs: int):
+        if not load_library().LBM_BatchSetBlockSize(self.ptr, bs):
+            raise RuntimeError("invalid block size")
+
+    @property
+    def ntunnels(self) -> int:
+        return int(load_library().LBM_BatchTunnelCount(self.ptr))
+
+    @property
+    def total_cells(self) -> int:
+        return int(load_library().LBM_BatchTotalCells(self.ptr))
+
+    def forces(self):
+        n = self.ntunnels
+        out = np.zeros((n, 6), np.float64)
+        fixed = np.zeros((n, 7), np.int64)
+        if not load_library().LBM_BatchComputeForcesF64(self.ptr, out.ctypes.data_as(c_double_p),
+                                                         fixed.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))):
+            raise RuntimeError(last_error())
+        return out, fixed
+
+    def coefficients(self) -> np.ndarray:
+        out = np.zeros((self.ntunnels, 4), np.float32)
+        if not load_library().LBM_BatchComputeCoefficients(self.ptr, _fp(out)):
+            raise RuntimeError(last_error())
+        return out
+
+    def populations(self, tunnel: int, cells: int):
+        f = np.empty((cells, 19), np.float64)
+        fn = np.empty((cells, 19), np.float64)
+        if not load_library().LBM_BatchCopyPopulationsToHostF64(self.ptr, tunnel, f.ctypes.data_as(c_double_p),
+                                                                 fn.ctypes.data_as(c_double_p)):
+            raise RuntimeError(last_error())
+        return f, fn
+
+    def velocity(self, tunnel: int, cells: int) -> np.ndarray:
+        out = np.empty((cells, 4), np.float32)
+        if not load_library().LBM_BatchCopyVelocityToHost(self.ptr, tunnel, _fp(out)):
+            raise RuntimeError(last_error())
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                load_library().LBM_BatchFree(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def LBM_MeasureCopyBandwidth(nbytes: int, iters: int = 20) -> float:
+    return float(load_library().LBM_MeasureCopyBandwidth(nbytes, iters))

@@ -30,16 +30,41 @@ def make_pair(nx, ny, nz, visc, setup, tau=None, precision=32, storage=lbm.LBM_S
 
 def setup_aabb(box):
     def f(g, og):
-        lbm.LBM_SetSolidAABB(g, *box)
+        if g is not None:
+            lbm.LBM_SetSolidAABB(g, *box)
         og.set_solid_aabb(*box)
     return f
 
 
 def setup_sphere(cx, cy, cz, r):
     def f(g, og):
-        lbm.LBM_SetSolidSphere(g, cx, cy, cz, r)
+        if g is not None:
+            lbm.LBM_SetSolidSphere(g, cx, cy, cz, r)
         og.set_solid_sphere(cx, cy, cz, r)
     return f
+
+
+def make_oracle(nx, ny, nz, visc, setup, tau=None, precision=64, u=0.05, **flags):
+    og = O.OracleGrid(nx, ny, nz, visc, precision=precision)
+    if tau is not None:
+        og.tau = float(np.float32(tau))
+    for k, v in flags.items():
+        setattr(og, k, v)
+    setup(None, og)
+    og.initialize_flow(u, 0, 0)
+    return og
+
+
+def fp64_report(g, og32, og64):
+    """Differences of the GPU (FP32) and of the FP32 oracle against the FP64
+    oracle, and GPU vs FP32 oracle, for u, rho and f."""
+    f, fn, vel = fields(g)
+    out = {}
+    for name, a, b, c in (("u", vel[:, :3], og32.vel[:, :3], og64.vel[:, :3]),
+                          ("rho", vel[:, 3], og32.vel[:, 3], og64.vel[:, 3]),
+                          ("f", f, og32.f, og64.f)):
+        out[name] = {"gpu_vs_fp64": rel_l2(a, c), "oracle32_vs_fp64": rel_l2(b, c), "gpu_vs_oracle32": rel_l2(a, b)}
+    return out
 
 
 def rel_l2(a, b):
