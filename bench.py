@@ -46,6 +46,11 @@ WORKLOADS = {
 DEFAULT = "roofline-512x256x256-regularized-smagorinsky"
 
 
+def bytes_per_cell_storage(w):
+    """Bytes of population storage per cell (the size of the probe copy)."""
+    return {0: 76, 1: 38, 2: 38, 3: 152}[w["storage"]]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -159,6 +164,8 @@ def main():
         parts = lbm.DeviceParticles(w["particles"])
         lbm.LBM_SeedParticles(g, parts, 0)
 
+    tuned_block, tune_ms = lbm.LBM_Autotune(g, 10, w["u"], 0.0, 0.0)
+    log(f"autotune: workgroup {tuned_block} (ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
     if args.warmup:
         run_frames(lbm, g, parts, w, args.warmup, 0, per_launch=False)
     wall, main_ms, bnd_ms, region_ms = run_frames(lbm, g, parts, w, args.steps, 10 ** 6,
@@ -170,6 +177,7 @@ def main():
     if not sane:
         raise RuntimeError("simulation state is not finite after the timed region")
 
+    copy_gbs = lbm.LBM_MeasureCopyBandwidth(int(bytes_per_cell_storage(w) * cells), 10)
     bpc = BYTES_PER_CELL[w["storage"]]
     roof = None
     if main_ms > 0:
@@ -179,6 +187,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload),
                 "kernel": "k_step (fused stream+collide)", "bytes_per_cell": bpc,
                 "avg_launch_ms": round(main_ms / args.steps, 5),
+                "copy_kernel_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
+                "workgroup": tuned_block,
                 "boundary_prepass_avg_ms": round(bnd_ms / max(args.steps, 1), 5)}
     cpu = None
     if args.cpu_steps > 0:

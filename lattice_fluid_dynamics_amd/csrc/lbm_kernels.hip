@@ -405,39 +405,43 @@ __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollidePa
         me[14] = rho * uy * uz;
         me[15] = rho * ux * uz;
         me[16] = me[17] = me[18] = T(0);
-        // relaxation rates (lbm_collide.comp:585-604), then scaling by
-        // 1/|row|^2 of the inverse (lbm_collide.comp:188-206)
+        // Relaxation rates (lbm_collide.comp:585-604).  f* = M^-1 (m - S (m - meq))
+        // is applied as f* = f - M^-1 S (m - meq): the conserved moments
+        // (rho, j; s = 0) drop out, so mass and momentum are conserved to
+        // unbiased rounding (scaling M^-1 by rounded reciprocals of the row
+        // norms would otherwise bias them by ~1e-8 per step).
         const T s[19] = {T(0),   T(1.19), T(1.4), T(0),   T(1.2), T(0),   T(1.2), T(0),    T(1.2),   omega,
                          T(1.4), omega,   T(1.4), omega,  omega,  omega,  T(1.98), T(1.98), T(1.98)};
+        // 1/|row|^2 of M (lbm_collide.comp:188-206)
         const T inrm[19] = {T(1) / T(19), T(1) / T(2394), T(1) / T(252), T(1) / T(10), T(1) / T(40),
                             T(1) / T(10), T(1) / T(40),   T(1) / T(10),  T(1) / T(40), T(1) / T(36),
                             T(1) / T(72), T(1) / T(12),   T(1) / T(24),  T(0.25),      T(0.25),
                             T(0.25),      T(0.125),       T(0.125),      T(0.125)};
-        T ms[19];
+        T c[19];
 #pragma unroll
-        for (int k = 0; k < 19; k++) ms[k] = (m[k] - s[k] * (m[k] - me[k])) * inrm[k];
-        const T b0 = ms[0] - T(11) * ms[1] - T(4) * ms[2];
-        const T b1 = ms[0] + T(8) * ms[1] + ms[2];
-        fo[0] = ms[0] - T(30) * ms[1] + T(12) * ms[2];
-        fo[1] = b0 + ms[3] - T(4) * ms[4] + T(2) * ms[9] - T(4) * ms[10];
-        fo[2] = b0 - ms[3] + T(4) * ms[4] + T(2) * ms[9] - T(4) * ms[10];
-        fo[3] = b0 + ms[5] - T(4) * ms[6] - ms[9] + T(2) * ms[10] + ms[11] - T(2) * ms[12];
-        fo[4] = b0 - ms[5] + T(4) * ms[6] - ms[9] + T(2) * ms[10] + ms[11] - T(2) * ms[12];
-        fo[5] = b0 + ms[7] - T(4) * ms[8] - ms[9] + T(2) * ms[10] - ms[11] + T(2) * ms[12];
-        fo[6] = b0 - ms[7] + T(4) * ms[8] - ms[9] + T(2) * ms[10] - ms[11] + T(2) * ms[12];
-        const T p910 = ms[9] + ms[10], p1112 = ms[11] + ms[12];
-        fo[7] = b1 + ms[3] + ms[4] + ms[5] + ms[6] + p910 + p1112 + ms[13] + ms[16] - ms[17];
-        fo[8] = b1 - ms[3] - ms[4] + ms[5] + ms[6] + p910 + p1112 - ms[13] - ms[16] - ms[17];
-        fo[9] = b1 + ms[3] + ms[4] - ms[5] - ms[6] + p910 + p1112 - ms[13] + ms[16] + ms[17];
-        fo[10] = b1 - ms[3] - ms[4] - ms[5] - ms[6] + p910 + p1112 + ms[13] - ms[16] + ms[17];
-        fo[11] = b1 + ms[3] + ms[4] + ms[7] + ms[8] + p910 - p1112 + ms[15] - ms[16] + ms[18];
-        fo[12] = b1 - ms[3] - ms[4] + ms[7] + ms[8] + p910 - p1112 - ms[15] + ms[16] + ms[18];
-        fo[13] = b1 + ms[3] + ms[4] - ms[7] - ms[8] + p910 - p1112 - ms[15] - ms[16] - ms[18];
-        fo[14] = b1 - ms[3] - ms[4] - ms[7] - ms[8] + p910 - p1112 + ms[15] + ms[16] - ms[18];
-        fo[15] = b1 + ms[5] + ms[6] + ms[7] + ms[8] - T(2) * p910 + ms[14] + ms[17] - ms[18];
-        fo[16] = b1 - ms[5] - ms[6] + ms[7] + ms[8] - T(2) * p910 - ms[14] - ms[17] - ms[18];
-        fo[17] = b1 + ms[5] + ms[6] - ms[7] - ms[8] - T(2) * p910 - ms[14] + ms[17] + ms[18];
-        fo[18] = b1 - ms[5] - ms[6] - ms[7] - ms[8] - T(2) * p910 + ms[14] - ms[17] + ms[18];
+        for (int k = 0; k < 19; k++) c[k] = s[k] * (m[k] - me[k]) * inrm[k];
+        const T b0 = -T(11) * c[1] - T(4) * c[2];
+        const T b1 = T(8) * c[1] + c[2];
+        const T p910 = c[9] + c[10], p1112 = c[11] + c[12];
+        fo[0] = f[0] - (-T(30) * c[1] + T(12) * c[2]);
+        fo[1] = f[1] - (b0 - T(4) * c[4] + T(2) * c[9] - T(4) * c[10]);
+        fo[2] = f[2] - (b0 + T(4) * c[4] + T(2) * c[9] - T(4) * c[10]);
+        fo[3] = f[3] - (b0 - T(4) * c[6] - c[9] + T(2) * c[10] + c[11] - T(2) * c[12]);
+        fo[4] = f[4] - (b0 + T(4) * c[6] - c[9] + T(2) * c[10] + c[11] - T(2) * c[12]);
+        fo[5] = f[5] - (b0 - T(4) * c[8] - c[9] + T(2) * c[10] - c[11] + T(2) * c[12]);
+        fo[6] = f[6] - (b0 + T(4) * c[8] - c[9] + T(2) * c[10] - c[11] + T(2) * c[12]);
+        fo[7] = f[7] - (b1 + c[4] + c[6] + p910 + p1112 + c[13] + c[16] - c[17]);
+        fo[8] = f[8] - (b1 - c[4] + c[6] + p910 + p1112 - c[13] - c[16] - c[17]);
+        fo[9] = f[9] - (b1 + c[4] - c[6] + p910 + p1112 - c[13] + c[16] + c[17]);
+        fo[10] = f[10] - (b1 - c[4] - c[6] + p910 + p1112 + c[13] - c[16] + c[17]);
+        fo[11] = f[11] - (b1 + c[4] + c[8] + p910 - p1112 + c[15] - c[16] + c[18]);
+        fo[12] = f[12] - (b1 - c[4] + c[8] + p910 - p1112 - c[15] + c[16] + c[18]);
+        fo[13] = f[13] - (b1 + c[4] - c[8] + p910 - p1112 - c[15] - c[16] - c[18]);
+        fo[14] = f[14] - (b1 - c[4] - c[8] + p910 - p1112 + c[15] + c[16] - c[18]);
+        fo[15] = f[15] - (b1 + c[6] + c[8] - T(2) * p910 + c[14] + c[17] - c[18]);
+        fo[16] = f[16] - (b1 - c[6] + c[8] - T(2) * p910 - c[14] - c[17] - c[18]);
+        fo[17] = f[17] - (b1 + c[6] - c[8] - T(2) * p910 - c[14] + c[17] + c[18]);
+        fo[18] = f[18] - (b1 - c[6] - c[8] - T(2) * p910 + c[14] - c[17] + c[18]);
     } else {  // BGK, lbm_collide.comp:618-625
 #pragma unroll
         for (int i = 0; i < 19; i++) fo[i] = f[i] - omega * (f[i] - feq_i<T>(i, rho, ux, uy, uz, c));
@@ -2134,11 +2138,18 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
 }
 
 // ------------------------------------------------------ copy bandwidth probe
-// A plain streaming copy (16 B per lane per iteration, grid-stride) used to
-// measure the attainable HBM bandwidth and to calibrate byte counters.
+// A plain streaming copy used to measure the attainable HBM bandwidth and
+// to calibrate byte counters: each lane moves four 16-byte vectors, one
+// workgroup per 16 KiB, no grid-stride loop.
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        dst[i] = src[i];
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (base + 256 * k < n) v[k] = src[base + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (base + 256 * k < n) dst[base + 256 * k] = v[k];
 }
 
 extern "C" int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float *ms_per_copy) {
@@ -2152,10 +2163,7 @@ extern "C" int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float
     }
     hipMemset(a, 0, (size_t)n * 16);
     hipMemset(b, 0, (size_t)n * 16);
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int grid = cus * 16;
+    const int grid = (int)((n + 1023) / 1024);
     hipEvent_t t0, t1;
     hipEventCreate(&t0);
     hipEventCreate(&t1);

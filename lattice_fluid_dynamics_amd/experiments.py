@@ -1,0 +1,148 @@
+"""Measured workloads beyond the roofline bench, each printing JSON lines.
+
+  python -m lattice_fluid_dynamics_amd.experiments storage [--steps 20000]
+      Config 2: 256x128x128 MRT + Smagorinsky (Cs 0.1), tau 0.52, inlet 0.06,
+      Ahmed-like body (64x24x18 cells, front radius 6, 25 deg slant, 2-cell
+      clearance, ~20k triangles, Bouzidi).  Cold starts with FP64, FP32, FP16
+      and BF16 population storage; Cd/Cl with the pressure/friction split, and
+      steps/seconds until the windowed Cd settles within 1 % of the FP64 value.
+
+  python -m lattice_fluid_dynamics_amd.experiments sweep [--tunnels 64 --steps 5000]
+      Config 4: tunnels with nx ~ U[64,192], ny = nz = nx/2, a seeded random box,
+      ellipsoid or slanted-rear body 10-20 % of ny, u ~ U[0.02,0.08],
+      tau ~ U[0.51,0.8]; MRT + Bouzidi + FP16 storage in one batch (one launch per
+      step, graph replay, autotuned workgroup); per-tunnel Cd/Cl.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import convergence, geometry, lbm
+
+
+def _emit(obj):
+    print(json.dumps(obj), flush=True)
+
+
+def config2_grid(storage: int):
+    nx, ny, nz = 256, 128, 128
+    g = lbm.LBM_Create(nx, ny, nz, (0.52 - 0.5) / 3)
+    g.tau = 0.52
+    g.useMRT = 1
+    g.useSmagorinsky = 1
+    g.smagorinskyCs = 0.1
+    g.periodicYZ = 1
+    g.populationStorage = storage
+    tris, bbox = geometry.ahmed_body(nx, ny, nz)
+    lbm.LBM_SetSolidMesh(g, tris, bbox)
+    area = lbm.LBM_ComputeProjectedArea(g, 0)
+    assert lbm.LBM_InitializeFlow(g, 0.06, 0, 0) == 1
+    return g, area, len(tris)
+
+
+def storage_experiment(steps: int, interval: int = 200):
+    names = {3: "fp64", 0: "fp32", 1: "fp16", 2: "bf16"}
+    hist = {}
+    for st in (3, 0, 1, 2):
+        g, area, ntri = config2_grid(st)
+        h = convergence.run_history(g, 0.06, area, steps, interval)
+        hist[st] = h
+        g.free()
+    ref = hist[3].converged_value()
+    for st, h in hist.items():
+        t = convergence.time_to_within(h, ref, 0.01, window=10)
+        tail = slice(-20, None)
+        cd = float(np.mean(h.cd[tail]))
+        cp = float(np.mean(h.cd_pressure[tail]))
+        _emit({"experiment": "config2-storage", "storage": names[st], "steps": steps, "triangles": ntri,
+               "area": area, "cd": cd, "cl": float(np.mean(h.cl[tail])), "cd_pressure": cp,
+               "cd_friction": max(cd - cp, 0.0), "cd_rel_err_vs_fp64": abs(cd - ref) / ref,
+               "steps_to_1pct_of_fp64": t, "seconds_to_1pct": None if t is None else t * h.ms_per_step / 1e3,
+               "ms_per_step": h.ms_per_step, "mlups": 256 * 128 * 128 / h.ms_per_step / 1e3})
+
+
+def sweep_bodies(seed: int, nx: int, ny: int, nz: int):
+    """Seeded random box, ellipsoid or slanted-rear body, 10-20 % of ny tall."""
+    rng = np.random.default_rng(seed)
+    kind = int(rng.integers(0, 3))
+    h = rng.uniform(0.10, 0.20) * 4.0  # world height (ny spans 4 world units)
+    L = h * rng.uniform(1.5, 2.5)
+    W = h * rng.uniform(0.8, 1.2)
+    c = np.array([-2.0 + rng.uniform(-0.3, 0.3), rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2)])
+    if kind == 0:
+        def r(d):
+            return geometry._box_radius(d, np.array([L, W, h]) / 2)
+    elif kind == 1:
+        def r(d):
+            return geometry._ellipsoid_radius(d, np.array([L, W, h]) / 2, np.zeros(3))
+    else:
+        ang = np.radians(rng.uniform(20, 35))
+        n_sl = np.array([np.sin(ang), 0.0, np.cos(ang)])
+        p0 = np.array([L / 2 - 0.4 * L, 0.0, h / 2])
+
+        def r(d):
+            box = geometry._box_radius(d, np.array([L, W, h]) / 2)
+            dn = d @ n_sl
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = np.where(dn > 1e-9, (p0 @ n_sl) / dn, np.inf)
+            return np.minimum(box, t)
+    tris = geometry.radial_mesh(r, c, 40, 40)
+    v = tris.reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
+    return tris, (*v.min(0).tolist(), *v.max(0).tolist()), ["box", "ellipsoid", "slanted-rear"][kind]
+
+
+def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 20):
+    rng = np.random.default_rng(12345)
+    b = lbm.Batch(storage=storage, useMRT=1, useSmagorinsky=0, periodicYZ=1)
+    meta = []
+    t0 = time.time()
+    for s in range(ntun):
+        nx = int(rng.integers(64, 193))
+        ny = nz = nx // 2
+        u = float(rng.uniform(0.02, 0.08))
+        tau = float(rng.uniform(0.51, 0.8))
+        g = lbm.LBM_Create(nx, ny, nz, (tau - 0.5) / 3)
+        tris, bbox, kind = sweep_bodies(s, nx, ny, nz)
+        lbm.LBM_SetSolidMesh(g, tris, bbox)
+        b.add(g, tau, u)
+        meta.append({"seed": s, "dims": [nx, ny, nz], "u": u, "tau": tau, "body": kind})
+        g.free()
+    b.initialize()
+    setup_s = time.time() - t0
+    cells = b.total_cells
+    best, ms = b.autotune(tune_steps)
+    ms_nograph = b.profile(20, use_graph=False) / 20
+    ms_graph = b.profile(20, use_graph=True) / 20
+    t = b.profile(steps, use_graph=True)
+    coef = b.coefficients()
+    _emit({"experiment": "config4-sweep", "tunnels": ntun, "total_cells": cells, "steps": steps,
+           "storage": {0: "fp32", 1: "fp16", 2: "bf16"}[storage], "setup_s": round(setup_s, 1),
+           "autotune_block": best, "autotune_ms_per_step": [round(float(v), 4) for v in ms],
+           "ms_per_step_eager": round(ms_nograph, 4), "ms_per_step_graph": round(ms_graph, 4),
+           "total_s": round(t / 1e3, 3), "aggregate_mlups": round(cells * steps / (t / 1e3) / 1e6, 1)})
+    for k, m in enumerate(meta):
+        m.update({"cd": float(coef[k, 0]), "cl": float(coef[k, 1]), "cd_pressure": float(coef[k, 2]),
+                  "cd_friction": float(coef[k, 3])})
+        _emit({"experiment": "config4-tunnel", **m})
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["storage", "sweep"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--tunnels", type=int, default=64)
+    ap.add_argument("--storage", type=int, default=1)
+    a = ap.parse_args(argv)
+    if a.which == "storage":
+        storage_experiment(a.steps or 20000)
+    else:
+        sweep_experiment(a.tunnels, a.steps or 5000, a.storage)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
