@@ -63,7 +63,8 @@ def storage_experiment(steps: int, interval: int = 200):
                "area": area, "cd": cd, "cl": float(np.mean(h.cl[tail])), "cd_pressure": cp,
                "cd_friction": max(cd - cp, 0.0), "cd_rel_err_vs_fp64": abs(cd - ref) / ref,
                "steps_to_1pct_of_fp64": t, "seconds_to_1pct": None if t is None else t * h.ms_per_step / 1e3,
-               "ms_per_step": h.ms_per_step, "mlups": 256 * 128 * 128 / h.ms_per_step / 1e3})
+               "ms_per_step": h.ms_per_step, "mlups": 256 * 128 * 128 / h.ms_per_step / 1e3,
+               "cd_history_every_1000": [round(v, 5) for v in h.cd[4::5]]})
 
 
 def sweep_bodies(seed: int, nx: int, ny: int, nz: int):

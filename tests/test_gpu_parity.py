@@ -196,7 +196,8 @@ def test_ground_plane_body_on_x_faces_and_clamp(gpu, face):
     assert_fp32_as_accurate(rep)
     ex_o, fix_o = og.force()
     F = gpu.LBM_ComputeDragForceF64(g)
-    assert np.allclose(F, ex_o, rtol=1e-4, atol=1e-6)
+    # per-cell rounding of sums of O(0.1) terms: absolute scale is |F|
+    assert np.allclose(F, ex_o, rtol=1e-4, atol=1e-5 * np.max(np.abs(ex_o)))
 
 
 def test_stability_guard_resets_cells(gpu):
@@ -291,22 +292,25 @@ def _case_setup(case, gpu):
 
 @pytest.mark.parametrize("case,periodic", [("sphere", 1), ("sphere", 0), ("inlet-body-ground", 0),
                                            ("inlet-body-ground", 1), ("outlet-body-ground", 0)])
-@pytest.mark.parametrize("op", ["bgk", "mrt", "reg"])
-def test_fp64_algorithm_exact_all_boundaries(gpu, case, periodic, op):
+@pytest.mark.parametrize("op,smag", [("bgk", 0), ("mrt", 0), ("reg", 0), ("mrt", 1), ("reg", 1)])
+def test_fp64_algorithm_exact_all_boundaries(gpu, case, periodic, op, smag):
     """In FP64 storage and arithmetic the GPU scheme (in-place two-phase
     streaming, boundary prepass, fused collision) reproduces the FP64 oracle to
-    relative L2 <= 1e-12 for every boundary configuration: the algorithm is
-    exact, and FP32 differences are rounding only."""
+    relative L2 <= 1e-12 for every boundary configuration (1e-10 with
+    Smagorinsky, whose square roots and divisions of small stresses amplify
+    FP64 rounding): the algorithm is exact, and FP32 differences are rounding
+    only.  A wrong stream rule or boundary slot shows up at 1e-6 or above."""
     kw = dict(tau=0.55, periodicYZ=periodic, useMRT=int(op == "mrt"), useRegularized=int(op == "reg"),
-              useSmagorinsky=int(op != "bgk"), smagorinskyCs=0.12)
+              useSmagorinsky=smag, smagorinskyCs=0.12)
+    bar = 1e-10 if smag else 1e-12
     setup = _case_setup(case, gpu)
     g, og = make_pair(40, 20, 20, 0.02, setup, precision=64, storage=gpu.LBM_STORAGE_FP64, **kw)
     gpu.LBM_StepN(g, 120, 0.05, 0, 0)
     og.step(0.05, 0, 0, n=120)
     f, fn = gpu.LBM_CopyPopulationsToHost(g, f64=True)
     d = (rel_l2(f, og.f), rel_l2(fn, og.f_new))
-    print(case, periodic, op, d)
-    assert d[0] < 1e-12 and d[1] < 1e-12
+    print(case, periodic, op, smag, d)
+    assert d[0] < bar and d[1] < bar
     ex_o, _ = og.force()
     F = gpu.LBM_ComputeDragForceF64(g)
     assert np.allclose(F[:3], ex_o[:3], rtol=1e-9, atol=1e-12)
