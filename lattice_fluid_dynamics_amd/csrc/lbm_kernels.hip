@@ -10,13 +10,6 @@
 //   phase B (odd):   cell x reads f_i from plane opp(i) at x - e_i, collides,
 //                    writes f*_i to plane i at x + e_i
 //
-// Within each x-row the slots that phase B writes (and phase A reads) are
-// stored rotated by -ex_i: logical x of plane i sits at physical
-// x - ex_i (wrapped in the row).  Both phases then store 64 consecutive
-// cells per wavefront on whole cache lines, and only loads are shifted by
-// one cell (their partial lines are L2 hits); unrotated, phase B's shifted
-// stores write partial lines twice (+10 % write traffic).
-//
 // Each cell reads and writes the same 19 slots within a phase, so the update
 // is race-free with one buffer: 76 B/cell of storage for FP32 and exactly
 // 152 B/cell/step of HBM traffic (19 loads + 19 stores).  Neighbour indices
@@ -184,13 +177,10 @@ __device__ __forceinline__ typename Pop<FMT>::T fstar(const typename Pop<FMT>::S
         int j = OPP[i];
         return P::dec(pop[(long long)j * g.stride + cidx(g, x, y, z)], j);
     } else if (STATE == ST_B) {
-        // logical slot x + e_i of plane i, stored rotated by -ex_i along x
-        int Y = wrapc(y + EY[i], g.ny), Z = wrapc(z + EZ[i], g.nz);
-        return P::dec(pop[(long long)i * g.stride + cidx(g, x, Y, Z)], i);
+        int X = wrapc(x + EX[i], g.nx), Y = wrapc(y + EY[i], g.ny), Z = wrapc(z + EZ[i], g.nz);
+        return P::dec(pop[(long long)i * g.stride + cidx(g, X, Y, Z)], i);
     } else {
-        // initial field, read by the first phase-A step in the rotated layout
-        int X = wrapc(x - EX[i], g.nx);
-        return P::dec(pop[(long long)i * g.stride + cidx(g, X, y, z)], i);
+        return P::dec(pop[(long long)i * g.stride + cidx(g, x, y, z)], i);
     }
 }
 
@@ -496,10 +486,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
     } else if (PHASE == 0) {
 #pragma unroll
-        for (int i = 0; i < 19; i++) {
-            const uint32_t src = (uint32_t)(X[1 - EX[i]] + Y[1] + Z[1]);
-            f[i] = P::dec(at(plane(a.pop, g, i), src), i);
-        }
+        for (int i = 0; i < 19; i++) f[i] = P::dec(at(plane(a.pop, g, i), (uint32_t)cell), i);
     } else {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
@@ -530,7 +517,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
     } else {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
-            const uint32_t dst = (uint32_t)(X[1] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
+            const uint32_t dst = (uint32_t)(X[1 + EX[i]] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
             at(plane(a.pop, g, i), dst) = P::enc(fo[i], i);
         }
     }
