@@ -81,6 +81,13 @@ void LBM_SetSolidAABB(LBMGrid *grid, float minX, float minY, float minZ, float m
 /* simulation/lib/lbm.h:74  f = f_new = feq(1, u), velocity = (u, 1). 1 on success, 0 on failure. */
 int LBM_InitializeFlow(LBMGrid *grid, float ux, float uy, float uz);
 
+/* No reference counterpart (warm starts): like LBM_InitializeFlow, but f =
+ * f_new = feq(rho, u) of a (ux, uy, uz, rho) field given on a coarse
+ * cell-centred lattice (host array, cnz x cny x cnx x 4 floats), trilinearly
+ * interpolated to every cell on the device; velocity = the interpolated
+ * field.  The inlet velocity passed to LBM_Step is unaffected. */
+int LBM_InitializeFlowFromField(LBMGrid *grid, const float *field, int cnx, int cny, int cnz);
+
 /* simulation/lib/lbm.h:77  one collide + stream step with the given inlet velocity. */
 void LBM_Step(LBMGrid *grid, float inletVelX, float inletVelY, float inletVelZ);
 
@@ -128,6 +135,9 @@ void LBM_StepN(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inl
 /* Replaces glGetBufferSubData on velocityBuffer (test_lbm.c:503-505,
  * vti_export.c:30-34): 4 floats per cell.  Returns 1 on success. */
 int LBM_CopyVelocityToHost(LBMGrid *grid, float *out);
+/* No reference counterpart (warm-start data): block means of the velocity
+ * buffer over factor^3 cells, (sizeZ/f) x (sizeY/f) x (sizeX/f) x 4 floats. */
+int LBM_CopyCoarseVelocityToHost(LBMGrid *grid, int factor, float *out);
 
 /* Replaces glGetBufferSubData on solidBuffer (test_lbm.c:258-260). */
 int LBM_CopySolidToHost(LBMGrid *grid, int *out);
@@ -162,6 +172,7 @@ void *LBM_DeviceAlloc(unsigned long long bytes);
 void LBM_DeviceFree(void *p);
 int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes);
 int LBM_CopyToHost(void *dst, const void *src, unsigned long long bytes);
+int LBM_CopyDeviceToDevice(void *dst, const void *src, unsigned long long bytes);
 
 /* Waits for all queued work of the grid. */
 void LBM_Synchronize(LBMGrid *grid);
@@ -231,6 +242,11 @@ int LBM_BatchAddTunnel(LBMBatch *batch, LBMGrid *geometry, float tau, float inle
                        float inletZ);
 /* Allocates every tunnel in one device buffer and sets f = feq(1, u_inlet). */
 int LBM_BatchInitialize(LBMBatch *batch);
+/* Replaces tunnel t's uniform initial state with the equilibrium of a coarse
+ * field, as LBM_InitializeFlowFromField; only between LBM_BatchInitialize
+ * and the first LBM_BatchStep. */
+int LBM_BatchInitializeTunnelFromField(LBMBatch *batch, int tunnel, const float *field, int cnx, int cny,
+                                       int cnz);
 /* n steps of every tunnel: one launch per step over all tunnels (plus the
  * boundary prepass), replayed from HIP graphs two steps at a time.  Each
  * tunnel evolves exactly as a single grid with the same settings would. */
@@ -247,6 +263,9 @@ int LBM_BatchComputeForcesF64(LBMBatch *batch, double *out, long long *fixedOut)
 int LBM_BatchComputeCoefficients(LBMBatch *batch, float *out);
 int LBM_BatchCopyPopulationsToHostF64(LBMBatch *batch, int tunnel, double *f, double *fNew);
 int LBM_BatchCopyVelocityToHost(LBMBatch *batch, int tunnel, float *out);
+/* Block means of the velocity buffer (ux, uy, uz, rho) over factor^3 cells:
+ * (nz/f) x (ny/f) x (nx/f) x 4 floats; the dimensions must divide by f. */
+int LBM_BatchCopyCoarseVelocityToHost(LBMBatch *batch, int tunnel, int factor, float *out);
 
 #ifdef __cplusplus
 }

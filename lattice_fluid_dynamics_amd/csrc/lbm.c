@@ -48,6 +48,9 @@ static void note_error(const char *msg) {
 
 const char *LBM_GetLastError(void) { return g_lastError; }
 
+/* For the other host-library files (warm start, batches). */
+void lbm_note_error(const char *msg) { note_error(msg); }
+
 static Impl *impl_of(LBMGrid *g) { return (Impl *)g->impl; }
 
 static long cell_index(const LBMGrid *g, int x, int y, int z) {
@@ -642,8 +645,8 @@ int LBM_GetBoundaryCellCount(LBMGrid *g) {
 
 /* ------------------------------------------------------------ flow */
 
-int LBM_InitializeFlow(LBMGrid *g, float ux, float uy, float uz) {
-    if (!g) return 0;
+/* Device buffers in the requested storage format and the geometry uploaded. */
+static int prepare_flow(LBMGrid *g) {
     Impl *im = impl_of(g);
     if (g->populationStorage != lbmdev_storage(im->dev)) {
         char err[256] = {0};
@@ -667,7 +670,13 @@ int LBM_InitializeFlow(LBMGrid *g, float ux, float uy, float uz) {
         g->solidBuffer = lbmdev_solid_ptr(im->dev);
         im->geomDirty = 1;
     }
-    if (!ensure_geometry(g)) return 0;
+    return ensure_geometry(g);
+}
+
+int LBM_InitializeFlow(LBMGrid *g, float ux, float uy, float uz) {
+    if (!g) return 0;
+    Impl *im = impl_of(g);
+    if (!prepare_flow(g)) return 0;
     float feq[19], v4[4] = {ux, uy, uz, 1.0f};
     for (int i = 0; i < 19; i++) feq[i] = host_feq(i, 1.0f, ux, uy, uz);
     if (!lbmdev_init_flow(im->dev, feq, v4)) {
@@ -677,6 +686,23 @@ int LBM_InitializeFlow(LBMGrid *g, float ux, float uy, float uz) {
     im->flowReady = 1;
     g->stepCount = 0;
     printf("LBM flow initialized: u=(%.3f, %.3f, %.3f)\n", ux, uy, uz);
+    return 1;
+}
+
+int LBM_InitializeFlowFromField(LBMGrid *g, const float *field, int cnx, int cny, int cnz) {
+    if (!g) return 0;
+    if (!field || cnx <= 0 || cny <= 0 || cnz <= 0 || cnx > g->sizeX || cny > g->sizeY || cnz > g->sizeZ) {
+        note_error("LBM_InitializeFlowFromField: the field must be a non-empty lattice no finer than the grid");
+        return 0;
+    }
+    Impl *im = impl_of(g);
+    if (!prepare_flow(g)) return 0;
+    if (!lbmdev_init_field(im->dev, field, cnx, cny, cnz)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    im->flowReady = 1;
+    g->stepCount = 0;
     return 1;
 }
 
@@ -807,6 +833,18 @@ int LBM_CopyVelocityToHost(LBMGrid *g, float *out) {
     return lbmdev_copy_velocity(impl_of(g)->dev, out);
 }
 
+int LBM_CopyCoarseVelocityToHost(LBMGrid *g, int factor, float *out) {
+    if (!g || !out || factor <= 0 || g->sizeX % factor || g->sizeY % factor || g->sizeZ % factor) {
+        note_error("LBM_CopyCoarseVelocityToHost: the grid must be a multiple of the factor on every axis");
+        return 0;
+    }
+    if (!lbmdev_copy_velocity_coarse(impl_of(g)->dev, factor, out)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    return 1;
+}
+
 int LBM_CopySolidToHost(LBMGrid *g, int *out) {
     if (!g || !out || !ensure_geometry(g)) return 0;
     return lbmdev_copy_solid(impl_of(g)->dev, (int32_t *)out);
@@ -860,6 +898,9 @@ int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes) {
 }
 int LBM_CopyToHost(void *dst, const void *src, unsigned long long bytes) {
     return lbmdev_memcpy_d2h(dst, src, (size_t)bytes);
+}
+int LBM_CopyDeviceToDevice(void *dst, const void *src, unsigned long long bytes) {
+    return lbmdev_memcpy_d2d(dst, src, (size_t)bytes);
 }
 
 /* Workgroup-size autotuner for the step kernel: times `steps` steps with each

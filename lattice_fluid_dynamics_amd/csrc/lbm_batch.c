@@ -183,6 +183,23 @@ int LBM_BatchInitialize(LBMBatch *b) {
     return ok;
 }
 
+int LBM_BatchInitializeTunnelFromField(LBMBatch *b, int t, const float *field, int cnx, int cny, int cnz) {
+    if (!b || !b->dev || !field || t < 0 || t >= b->nt || cnx <= 0 || cny <= 0 || cnz <= 0 || cnx > b->t[t].nx ||
+        cny > b->t[t].ny || cnz > b->t[t].nz) {
+        batch_error("LBM_BatchInitializeTunnelFromField: bad arguments");
+        return 0;
+    }
+    if (b->stepCount != 0) {
+        batch_error("LBM_BatchInitializeTunnelFromField: the batch has stepped since LBM_BatchInitialize");
+        return 0;
+    }
+    if (!lbmbatch_init_tunnel_field(b->dev, t, field, cnx, cny, cnz)) {
+        batch_error(lbmdev_last_error());
+        return 0;
+    }
+    return 1;
+}
+
 static void params_of(const LBMBatch *b, LbmBatchParams *p) {
     p->smagorinskyCs = b->smagorinskyCs;
     p->useRegularized = b->useRegularized;
@@ -268,4 +285,13 @@ int LBM_BatchCopyPopulationsToHostF64(LBMBatch *b, int tunnel, double *f, double
 int LBM_BatchCopyVelocityToHost(LBMBatch *b, int tunnel, float *out) {
     if (!b || !b->dev) return 0;
     return lbmbatch_copy_velocity(b->dev, tunnel, out);
+}
+
+int LBM_BatchCopyCoarseVelocityToHost(LBMBatch *b, int tunnel, int factor, float *out) {
+    if (!b || !b->dev || !out || tunnel < 0 || tunnel >= b->nt || factor <= 0 || b->t[tunnel].nx % factor ||
+        b->t[tunnel].ny % factor || b->t[tunnel].nz % factor) {
+        batch_error("LBM_BatchCopyCoarseVelocityToHost: bad arguments");
+        return 0;
+    }
+    return lbmbatch_copy_velocity_coarse(b->dev, tunnel, factor, out);
 }

@@ -45,6 +45,8 @@ int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, const int32_
 
 /* f = f_new = feq (19 host-computed values) everywhere, vel = vel4. */
 int lbmdev_init_flow(LbmDev *d, const float feq[19], const float vel4[4]);
+/* equilibrium of a coarse (ux, uy, uz, rho) host field, trilinear upsampling */
+int lbmdev_init_field(LbmDev *d, const float *coarse_host, int cnx, int cny, int cnz);
 
 /* n LBM steps; the velocity buffer is written on the last one. */
 int lbmdev_step(LbmDev *d, const LbmStepParams *p, int nsteps);
@@ -57,6 +59,8 @@ int lbmdev_force(LbmDev *d, int periodicYZ, double exact[6], long long fixed[7])
 float *lbmdev_velocity_ptr(LbmDev *d);
 int32_t *lbmdev_solid_ptr(LbmDev *d);
 int lbmdev_copy_velocity(LbmDev *d, float *host);
+/* block means over f^3 cells (dimensions divisible by f) */
+int lbmdev_copy_velocity_coarse(LbmDev *d, int f, float *host);
 int lbmdev_copy_solid(LbmDev *d, int32_t *host);
 /* Reference-layout views (f[i + 19*cell]): post-stream f and post-collision
  * f_new of the last step, as FP32 (either may be NULL). */
@@ -99,6 +103,7 @@ void lbmbatch_destroy(LbmBatchDev *b);
 int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const int *n_ovr_per, const int32_t *ovr_cells,
                           const float *ovr_q);
 int lbmbatch_init_flow(LbmBatchDev *b, const float *feq, const float *vel4);
+int lbmbatch_init_tunnel_field(LbmBatchDev *b, int t, const float *coarse_host, int cnx, int cny, int cnz);
 int lbmbatch_step(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph);
 int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph, float *ms);
 int lbmbatch_set_block_size(LbmBatchDev *b, int bs);
@@ -107,6 +112,7 @@ int lbmbatch_block_size(const LbmBatchDev *b);
 int lbmbatch_force(LbmBatchDev *b, int periodic, double *exact, long long *fixed);
 int lbmbatch_export(LbmBatchDev *b, int t, int periodic, double *f_post_stream, double *f_post_collision);
 int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host);
+int lbmbatch_copy_velocity_coarse(LbmBatchDev *b, int t, int f, float *host);
 int lbmbatch_synchronize(LbmBatchDev *b);
 
 /* Read + write bandwidth of a plain copy of `bytes` bytes (mean of iters). */
@@ -116,7 +122,15 @@ void *lbmdev_malloc(size_t bytes);
 void lbmdev_free(void *p);
 int lbmdev_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int lbmdev_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int lbmdev_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int lbmdev_device_count(void);
+
+/* lbm.c: records the message returned by LBM_GetLastError */
+void lbm_note_error(const char *msg);
+
+/* warmstart_kernels.hip: device helpers of the warm-start network */
+int lbmnn_memset(void *p, int value, unsigned long long bytes);
+int lbmnn_synchronize(void);
 
 #ifdef __cplusplus
 }

@@ -839,6 +839,43 @@ __global__ __launch_bounds__(BLOCK) void k_init(typename Pop<FMT>::S *pop, Geo g
     vel[cell] = v4;
 }
 
+// Initial populations at equilibrium with a (ux, uy, uz, rho) field given on
+// a coarse cell-centred lattice of cnx x cny x cnz cells, trilinearly
+// interpolated: fine cell x sits at coarse coordinate (x + 1/2) cnx/nx - 1/2,
+// clamped to the coarse cell centres at the edges.  Used by warm starts.
+template <int FMT>
+__global__ __launch_bounds__(BLOCK) void k_init_field(typename Pop<FMT>::S *pop, Geo g, float4 *vel,
+                                                      const float4 *__restrict__ coarse, int cnx, int cny,
+                                                      int cnz) {
+    typedef typename Pop<FMT>::T T;
+    const int cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= g.ncell) return;
+    int x, y, z;
+    decompose(g, cell, x, y, z);
+    const float gx = fminf(fmaxf(((float)x + 0.5f) * (float)cnx / (float)g.nx - 0.5f, 0.0f), (float)(cnx - 1));
+    const float gy = fminf(fmaxf(((float)y + 0.5f) * (float)cny / (float)g.ny - 0.5f, 0.0f), (float)(cny - 1));
+    const float gz = fminf(fmaxf(((float)z + 0.5f) * (float)cnz / (float)g.nz - 0.5f, 0.0f), (float)(cnz - 1));
+    const int x0 = (int)gx, y0 = (int)gy, z0 = (int)gz;
+    const int x1 = min(x0 + 1, cnx - 1), y1 = min(y0 + 1, cny - 1), z1 = min(z0 + 1, cnz - 1);
+    const float fx = gx - (float)x0, fy = gy - (float)y0, fz = gz - (float)z0;
+    const long long sy = cnx, sz = (long long)cnx * cny;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float w = ((k & 1) ? fx : 1.0f - fx) * ((k & 2) ? fy : 1.0f - fy) * ((k & 4) ? fz : 1.0f - fz);
+        const float4 c = coarse[((k & 1) ? x1 : x0) + ((k & 2) ? y1 : y0) * sy + ((k & 4) ? z1 : z0) * sz];
+        v.x += w * c.x;
+        v.y += w * c.y;
+        v.z += w * c.z;
+        v.w += w * c.w;
+    }
+    const T ux = (T)v.x, uy = (T)v.y, uz = (T)v.z, rho = (T)v.w;
+    const T c = T(1) - T(1.5) * (ux * ux + uy * uy + uz * uz);
+#pragma unroll
+    for (int i = 0; i < 19; i++) pop[(long long)i * g.stride + cell] = Pop<FMT>::enc(feq_i<T>(i, rho, ux, uy, uz, c), i);
+    vel[cell] = v;
+}
+
 // ------------------------------------------------------------ particles
 struct Particle {
     float x, y, z, pad, vx, vy, vz, life;
@@ -1148,6 +1185,44 @@ extern "C" int lbmdev_init_flow(LbmDev *d, const float feq[19], const float vel4
 }
 
 template <int FMT>
+static void init_field_t(typename Pop<FMT>::S *pop, const Geo &g, float4 *vel, const float *coarse, int cnx, int cny,
+                         int cnz, hipStream_t s) {
+    const int nb = (int)((g.ncell + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_init_field<FMT>, dim3(nb), dim3(BLOCK), 0, s, pop, g, vel, (const float4 *)coarse, cnx,
+                       cny, cnz);
+}
+
+static void init_field_dispatch(int fmt, void *pop, const Geo &g, float4 *vel, const float *coarse, int cnx, int cny,
+                                int cnz, hipStream_t s) {
+    switch (fmt) {
+    case LBMDEV_FP32: init_field_t<LBMDEV_FP32>((float *)pop, g, vel, coarse, cnx, cny, cnz, s); break;
+    case LBMDEV_FP16S: init_field_t<LBMDEV_FP16S>((uint16_t *)pop, g, vel, coarse, cnx, cny, cnz, s); break;
+    case LBMDEV_BF16S: init_field_t<LBMDEV_BF16S>((uint16_t *)pop, g, vel, coarse, cnx, cny, cnz, s); break;
+    case LBMDEV_FP64: init_field_t<LBMDEV_FP64>((double *)pop, g, vel, coarse, cnx, cny, cnz, s); break;
+    }
+}
+
+// Equilibrium initial state from a coarse (ux, uy, uz, rho) field on the host
+// (cnz x cny x cnx x 4 floats), upsampled on the device.
+extern "C" int lbmdev_init_field(LbmDev *d, const float *coarse_host, int cnx, int cny, int cnz) {
+    const size_t bytes = sizeof(float) * 4 * (size_t)cnx * cny * cnz;
+    float *c = NULL;
+    HIPCHECK(hipMalloc((void **)&c, bytes), "coarse field");
+    hipError_t e = hipMemcpy(c, coarse_host, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        init_field_dispatch(d->fmt, d->pop, make_geo(d->nx, d->ny, d->nz, d->stride), d->vel, c, cnx, cny, cnz,
+                            d->stream);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(d->stream);
+    hipFree(c);
+    if (e != hipSuccess) return set_err("field init", e);
+    if (e2 != hipSuccess) return set_err("field init", e2);
+    d->state = ST_INIT;
+    return 1;
+}
+
+template <int FMT>
 static void launch_prepass(LbmDev *d, int periodic) {
     if (d->n_ovr == 0) return;
     BoundArgs<FMT> a;
@@ -1444,6 +1519,47 @@ extern "C" int lbmdev_copy_velocity(LbmDev *d, float *host) {
     return 1;
 }
 
+// Mean of the velocity buffer over factor^3 blocks: coarse cell (X, Y, Z)
+// averages fine cells [fX, fX + f) x [fY, fY + f) x [fZ, fZ + f).
+__global__ __launch_bounds__(BLOCK) void k_coarsen(const float4 *__restrict__ vel, int nx, int ny, int f,
+                                                   float4 *__restrict__ out, int cnx, int cny, int ncoarse) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= ncoarse) return;
+    const int X = c % cnx, Y = (c / cnx) % cny, Z = c / (cnx * cny);
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int dz = 0; dz < f; dz++)
+        for (int dy = 0; dy < f; dy++) {
+            const long long row = ((long long)(Z * f + dz) * ny + (Y * f + dy)) * nx + (long long)X * f;
+            for (int dx = 0; dx < f; dx++) {
+                const float4 v = vel[row + dx];
+                s.x += v.x;
+                s.y += v.y;
+                s.z += v.z;
+                s.w += v.w;
+            }
+        }
+    const float inv = 1.0f / (float)(f * f * f);
+    out[c] = make_float4(s.x * inv, s.y * inv, s.z * inv, s.w * inv);
+}
+
+static int coarsen_to_host(const float4 *vel, int nx, int ny, int nz, int f, float *host, hipStream_t s) {
+    const int cnx = nx / f, cny = ny / f, cnz = nz / f, nc = cnx * cny * cnz;
+    float4 *c = NULL;
+    HIPCHECK(hipMalloc((void **)&c, sizeof(float4) * (size_t)nc), "coarse velocity");
+    hipLaunchKernelGGL(k_coarsen, dim3((nc + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, vel, nx, ny, f, c, cnx, cny, nc);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, c, sizeof(float4) * (size_t)nc, hipMemcpyDeviceToHost, s);
+    hipError_t e2 = hipStreamSynchronize(s);
+    hipFree(c);
+    if (e != hipSuccess) return set_err("coarse velocity", e);
+    if (e2 != hipSuccess) return set_err("coarse velocity", e2);
+    return 1;
+}
+
+extern "C" int lbmdev_copy_velocity_coarse(LbmDev *d, int f, float *host) {
+    return coarsen_to_host(d->vel, d->nx, d->ny, d->nz, f, host, d->stream);
+}
+
 extern "C" int lbmdev_copy_solid(LbmDev *d, int32_t *host) {
     HIPCHECK(hipMemcpyAsync(host, d->solid, (size_t)d->ncell * 4, hipMemcpyDeviceToHost, d->stream), "solid readback");
     HIPCHECK(hipStreamSynchronize(d->stream), "solid readback");
@@ -1492,6 +1608,11 @@ extern "C" int lbmdev_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 
 extern "C" int lbmdev_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copy to host");
+    return 1;
+}
+
+extern "C" int lbmdev_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "device copy");
     return 1;
 }
 
@@ -1768,6 +1889,29 @@ extern "C" int lbmbatch_init_flow(LbmBatchDev *b, const float *feq, const float 
     return 1;
 }
 
+// Overwrites tunnel t of a freshly initialised batch with the equilibrium of
+// a coarse field (see lbmdev_init_field).
+extern "C" int lbmbatch_init_tunnel_field(LbmBatchDev *b, int t, const float *coarse_host, int cnx, int cny,
+                                          int cnz) {
+    if (t < 0 || t >= b->nt) return set_err("field init: tunnel index", hipErrorInvalidValue);
+    if (b->state != ST_INIT) return set_err("field init: batch must be freshly initialised", hipErrorInvalidValue);
+    const size_t bytes = sizeof(float) * 4 * (size_t)cnx * cny * cnz;
+    float *c = NULL;
+    HIPCHECK(hipMalloc((void **)&c, bytes), "coarse field");
+    hipError_t e = hipMemcpy(c, coarse_host, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const TunnelDesc &d = b->desc_h[t];
+        init_field_dispatch(b->fmt, (char *)b->pop + (size_t)d.popOff * b->elem, d.g, b->vel + d.cellOff, c, cnx, cny,
+                            cnz, b->stream);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(b->stream);
+    hipFree(c);
+    if (e != hipSuccess) return set_err("field init", e);
+    if (e2 != hipSuccess) return set_err("field init", e2);
+    return 1;
+}
+
 template <int FMT, int OP, int BS>
 static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase, int use_ovr, int wv) {
     BatchArgs<FMT> a;
@@ -2033,6 +2177,12 @@ extern "C" int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host) {
              "velocity readback");
     HIPCHECK(hipStreamSynchronize(b->stream), "velocity readback");
     return 1;
+}
+
+extern "C" int lbmbatch_copy_velocity_coarse(LbmBatchDev *b, int t, int f, float *host) {
+    if (t < 0 || t >= b->nt) return 0;
+    const TunnelDesc &d = b->desc_h[t];
+    return coarsen_to_host(b->vel + d.cellOff, d.g.nx, d.g.ny, d.g.nz, f, host, b->stream);
 }
 
 extern "C" int lbmbatch_synchronize(LbmBatchDev *b) {

@@ -1,0 +1,441 @@
+// Kernels of the warm-start network: a small 3D convolutional encoder-decoder
+// trained and run in BF16 on the matrix cores of gfx950, FP32 accumulation.
+//
+// Tensors are channels-last (N, D, H, W, C) BF16.  A 3x3x3 convolution
+// (stride 1, zero padding 1) is an implicit GEMM:
+//   rows    = output positions (N*D*H*W),
+//   columns = output channels,
+//   K       = 27 taps x input channels,
+// computed with v_mfma_f32_16x16x32_bf16.  Operand layout of that
+// instruction (checked on the device): lane l holds A[l%16][8*(l/16)+j] and
+// B[8*(l/16)+j][l%16] for j = 0..7, and returns D[4*(l/16)+j][l%16], j = 0..3.
+//
+// The input-gradient (dgrad) of a convolution is the same forward kernel run
+// on the output gradient with the weights mirrored in space and transposed
+// in channels; the weight gradient (wgrad) has its own MFMA kernel with the
+// positions as the K dimension.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef uint16_t bf16_t;  // storage
+
+namespace {
+
+__device__ __forceinline__ float bf2f(bf16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16_t)(u >> 16);
+}
+
+struct Shape {
+    int n, d, h, w;  // batch and spatial extent
+};
+
+__device__ __forceinline__ bool tap_pos(const Shape &s, int p, int t, int &q) {
+    // p = ((n*D + z)*H + y)*W + x ; tap t = (dz+1)*9 + (dy+1)*3 + (dx+1)
+    const int x = p % s.w;
+    int r = p / s.w;
+    const int y = r % s.h;
+    r /= s.h;
+    const int z = r % s.d;
+    const int n = r / s.d;
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    const int X = x + dx, Y = y + dy, Z = z + dz;
+    if (X < 0 || X >= s.w || Y < 0 || Y >= s.h || Z < 0 || Z >= s.d) return false;
+    q = ((n * s.d + Z) * s.h + Y) * s.w + X;
+    return true;
+}
+
+union Pack8 {
+    uint4 u;
+    bf16x8 v;
+};
+
+// ---------------------------------------------------------------- forward
+// Y[p][co] = act(bias[co] + sum_{t, ci} W[co][t][ci] * X[p + off(t)][ci])
+// Workgroup: 4 waves x 32 positions = 128 positions; NB column blocks of 16
+// output channels (blockIdx.y selects the 16*NB-channel tile).
+// act: 0 none, 1 ReLU.  Yf (optional) receives the FP32 result.
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
+                                                  const float *__restrict__ bias, bf16_t *__restrict__ Y,
+                                                  float *__restrict__ Yf, Shape s, int cin, int cout, int act) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = s.n * s.d * s.h * s.w;
+    const int p0 = (blockIdx.x * 4 + wave) * 32;
+    const int co0 = blockIdx.y * 16 * NB;
+    const int r = lane & 15, kq = lane >> 4;  // row/column within a block, k quarter
+    f32x4 acc[2][NB];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    const int prow[2] = {p0 + r, p0 + 16 + r};
+    for (int t = 0; t < 27; t++) {
+        int q[2];
+        bool ok[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            q[a] = 0;
+            ok[a] = prow[a] < P && tap_pos(s, prow[a], t, q[a]);
+        }
+        for (int c0 = 0; c0 < cin; c0 += 32) {
+            Pack8 A[2], B[NB];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                A[a].u = make_uint4(0, 0, 0, 0);
+                if (ok[a]) A[a].u = *(const uint4 *)(X + (long long)q[a] * cin + c0 + kq * 8);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int co = co0 + 16 * b + r;
+                B[b].u = make_uint4(0, 0, 0, 0);
+                if (co < cout) B[b].u = *(const uint4 *)(Wt + ((long long)co * 27 + t) * cin + c0 + kq * 8);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a].v, B[b].v, acc[a][b], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int co = co0 + 16 * b + r;
+            if (co >= cout) continue;
+            const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int p = p0 + 16 * a + 4 * kq + j;
+                if (p >= P) continue;
+                float v = acc[a][b][j] + bv;
+                if (act == 1) v = fmaxf(v, 0.0f);
+                Y[(long long)p * cout + co] = f2bf(v);
+                if (Yf) Yf[(long long)p * cout + co] = v;
+            }
+        }
+}
+
+// W[co][t][ci] -> Wm[ci][26 - t][co] (mirror taps, swap channels): the
+// weights of the input-gradient convolution.
+__global__ void k_conv_mirror(const bf16_t *W, bf16_t *Wm, int cin, int cout) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)cout * 27 * cin;
+    if (i >= n) return;
+    const int ci = (int)(i % cin);
+    const int t = (int)((i / cin) % 27);
+    const int co = (int)(i / ((long long)cin * 27));
+    Wm[((long long)ci * 27 + (26 - t)) * cout + co] = W[i];
+}
+
+// ---------------------------------------------------------------- wgrad
+// dW[co][t][ci] += sum_p dZ[p][co] * X[p + off(t)][ci] over a chunk of
+// positions per workgroup (K = positions), FP32 atomics into dW.
+// Workgroup: 4 waves, each a 16x16 (co x ci) block of one tap; grid
+// (position chunks, 27 taps, co/16 * ci/16 blocks / 4).
+__global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t *__restrict__ dZ, const bf16_t *__restrict__ X,
+                                                    float *__restrict__ dW, Shape s, int cin, int cout,
+                                                    int chunk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = s.n * s.d * s.h * s.w;
+    const int t = blockIdx.y;
+    const int nbci = cin / 16;
+    const int blk = blockIdx.z * 4 + wave;
+    if (blk >= (cout / 16) * nbci) return;
+    const int co0 = (blk / nbci) * 16, ci0 = (blk % nbci) * 16;
+    const int r = lane & 15, kq = lane >> 4;
+    const int pbeg = blockIdx.x * chunk, pend = min(P, pbeg + chunk);
+    f32x4 acc = f32x4{0, 0, 0, 0};
+    for (int pk = pbeg; pk < pend; pk += 32) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int p = pk + 8 * kq + j;
+            bf16_t av = 0, bv = 0;
+            if (p < pend) {
+                av = dZ[(long long)p * cout + co0 + r];
+                int q;
+                if (tap_pos(s, p, t, q)) bv = X[(long long)q * cin + ci0 + r];
+            }
+            a[j] = __builtin_bit_cast(__bf16, av);
+            b[j] = __builtin_bit_cast(__bf16, bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    // D[row = co][col = ci]
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int co = co0 + 4 * kq + j, ci = ci0 + r;
+        atomicAdd(&dW[((long long)co * 27 + t) * cin + ci], acc[j]);
+    }
+}
+
+// ------------------------------------------------------------ elementwise
+// dZ = dY * (Y > 0)  (ReLU backward on the stored activation), in place on dY
+__global__ void k_relu_bwd(bf16_t *dY, const bf16_t *Y, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!(bf2f(Y[i]) > 0.0f)) dY[i] = 0;
+}
+
+// db[c] += sum over positions of dZ[p][c]
+__global__ void k_bias_grad(const bf16_t *dZ, float *db, long long P, int c) {
+    const int ch = blockIdx.y * blockDim.x + threadIdx.x;
+    if (ch >= c) return;
+    float s = 0;
+    for (long long p = blockIdx.x; p < P; p += gridDim.x) s += bf2f(dZ[p * c + ch]);
+    atomicAdd(&db[ch], s);
+}
+
+// 2x2x2 average pooling: Y (D/2, H/2, W/2) from X
+__global__ void k_pool_fwd(const bf16_t *X, bf16_t *Y, Shape so, int c) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)so.n * so.d * so.h * so.w * c;
+    if (i >= n) return;
+    const int ch = (int)(i % c);
+    long long p = i / c;
+    const int x = (int)(p % so.w);
+    p /= so.w;
+    const int y = (int)(p % so.h);
+    p /= so.h;
+    const int z = (int)(p % so.d);
+    const int b = (int)(p / so.d);
+    const int W = 2 * so.w, H = 2 * so.h, D = 2 * so.d;
+    float s = 0;
+    for (int k = 0; k < 8; k++) {
+        const int Z = 2 * z + (k >> 2), Y2 = 2 * y + ((k >> 1) & 1), X2 = 2 * x + (k & 1);
+        s += bf2f(X[((((long long)b * D + Z) * H + Y2) * W + X2) * c + ch]);
+    }
+    Y[i] = f2bf(s * 0.125f);
+}
+
+// average-pool backward: dX (fine) = dY (coarse) / 8
+__global__ void k_pool_bwd(const bf16_t *dY, bf16_t *dX, Shape sf, int c) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)sf.n * sf.d * sf.h * sf.w * c;
+    if (i >= n) return;
+    const int ch = (int)(i % c);
+    long long p = i / c;
+    const int x = (int)(p % sf.w);
+    p /= sf.w;
+    const int y = (int)(p % sf.h);
+    p /= sf.h;
+    const int z = (int)(p % sf.d);
+    const int b = (int)(p / sf.d);
+    const long long q = ((((long long)b * (sf.d / 2) + z / 2) * (sf.h / 2) + y / 2) * (sf.w / 2) + x / 2);
+    dX[i] = f2bf(bf2f(dY[q * c + ch]) * 0.125f);
+}
+
+// nearest x2 upsampling: Y (fine) from X (coarse); optional skip added
+__global__ void k_up_fwd(const bf16_t *X, const bf16_t *skip, bf16_t *Y, Shape sf, int c) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)sf.n * sf.d * sf.h * sf.w * c;
+    if (i >= n) return;
+    const int ch = (int)(i % c);
+    long long p = i / c;
+    const int x = (int)(p % sf.w);
+    p /= sf.w;
+    const int y = (int)(p % sf.h);
+    p /= sf.h;
+    const int z = (int)(p % sf.d);
+    const int b = (int)(p / sf.d);
+    const long long q = ((((long long)b * (sf.d / 2) + z / 2) * (sf.h / 2) + y / 2) * (sf.w / 2) + x / 2);
+    float v = bf2f(X[q * c + ch]);
+    if (skip) v += bf2f(skip[i]);
+    Y[i] = f2bf(v);
+}
+
+// upsampling backward: dX (coarse) = sum over the 8 children of dY (fine)
+__global__ void k_up_bwd(const bf16_t *dY, bf16_t *dX, Shape so, int c) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)so.n * so.d * so.h * so.w * c;
+    if (i >= n) return;
+    const int ch = (int)(i % c);
+    long long p = i / c;
+    const int x = (int)(p % so.w);
+    p /= so.w;
+    const int y = (int)(p % so.h);
+    p /= so.h;
+    const int z = (int)(p % so.d);
+    const int b = (int)(p / so.d);
+    const int W = 2 * so.w, H = 2 * so.h, D = 2 * so.d;
+    float s = 0;
+    for (int k = 0; k < 8; k++) {
+        const int Z = 2 * z + (k >> 2), Y2 = 2 * y + ((k >> 1) & 1), X2 = 2 * x + (k & 1);
+        s += bf2f(dY[((((long long)b * D + Z) * H + Y2) * W + X2) * c + ch]);
+    }
+    dX[i] = f2bf(s);
+}
+
+// a += b (gradient accumulation into a skip branch)
+__global__ void k_add_inplace(bf16_t *a, const bf16_t *b, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+
+// MSE loss on FP32 predictions: dY = 2 (Y - T) / n (BF16), loss partials
+__global__ void k_mse(const float *Yf, const float *T, bf16_t *dY, float *loss, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0;
+    if (i < n) {
+        const float d = Yf[i] - T[i];
+        dY[i] = f2bf(2.0f * d / (float)n);
+        l = d * d / (float)n;
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+}
+
+// AdamW on FP32 master weights, refreshing the BF16 copy (decoupled decay).
+__global__ void k_adamw(float *w, bf16_t *wb, const float *g, float *m, float *v, long long n, float lr, float b1,
+                        float b2, float eps, float wd, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float wi = w[i] * (1.0f - lr * wd);
+    wi -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void k_f32_to_bf16(const float *a, bf16_t *b, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+inline dim3 g1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+// ======================================================================
+// C ABI (device pointers; work on the default stream, caller synchronises)
+// ======================================================================
+extern "C" {
+
+static int ws_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "warm-start %s: %s\n", what, hipGetErrorString(e));
+        return 0;
+    }
+    return 1;
+}
+
+// cin % 32 == 0, cout % 16 == 0.
+int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, float *Yf, int n, int d, int h, int w,
+                   int cin, int cout, int relu) {
+    if (cin % 32 != 0 || cout % 16 != 0) return 0;
+    Shape s{n, d, h, w};
+    const long long P = (long long)n * d * h * w;
+    const int nbTotal = cout / 16;
+    const int NB = nbTotal >= 4 ? 4 : nbTotal;
+    dim3 grid((unsigned)((P + 127) / 128), (unsigned)((nbTotal + NB - 1) / NB));
+    const bf16_t *x = (const bf16_t *)X, *wt = (const bf16_t *)W;
+    bf16_t *y = (bf16_t *)Y;
+    switch (NB) {
+    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
+    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
+    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
+    default: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
+    }
+    return ws_check("conv forward");
+}
+
+int lbmnn_conv_mirror(const void *W, void *Wm, int cin, int cout) {
+    const long long n = (long long)cout * 27 * cin;
+    hipLaunchKernelGGL(k_conv_mirror, g1(n), dim3(256), 0, 0, (const bf16_t *)W, (bf16_t *)Wm, cin, cout);
+    return ws_check("weight mirror");
+}
+
+// dW (FP32, zeroed by the caller) += weight gradient; cin, cout % 16 == 0.
+int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, int n, int d, int h, int w, int cin, int cout) {
+    if (cin % 16 != 0 || cout % 16 != 0) return 0;
+    Shape s{n, d, h, w};
+    const long long P = (long long)n * d * h * w;
+    const int chunk = 1024;
+    const int blocks = (cout / 16) * (cin / 16);
+    dim3 grid((unsigned)((P + chunk - 1) / chunk), 27, (unsigned)((blocks + 3) / 4));
+    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, 0, (const bf16_t *)dZ, (const bf16_t *)X, dW, s, cin, cout,
+                       chunk);
+    return ws_check("conv wgrad");
+}
+
+int lbmnn_relu_bwd(void *dY, const void *Y, long long n) {
+    hipLaunchKernelGGL(k_relu_bwd, g1(n), dim3(256), 0, 0, (bf16_t *)dY, (const bf16_t *)Y, n);
+    return ws_check("relu backward");
+}
+
+int lbmnn_bias_grad(const void *dZ, float *db, long long P, int c) {
+    dim3 grid(256, (unsigned)((c + 63) / 64));
+    hipLaunchKernelGGL(k_bias_grad, grid, dim3(64), 0, 0, (const bf16_t *)dZ, db, P, c);
+    return ws_check("bias gradient");
+}
+
+int lbmnn_pool_fwd(const void *X, void *Y, int n, int dOut, int hOut, int wOut, int c) {
+    Shape so{n, dOut, hOut, wOut};
+    const long long m = (long long)n * dOut * hOut * wOut * c;
+    hipLaunchKernelGGL(k_pool_fwd, g1(m), dim3(256), 0, 0, (const bf16_t *)X, (bf16_t *)Y, so, c);
+    return ws_check("pool forward");
+}
+
+int lbmnn_pool_bwd(const void *dY, void *dX, int n, int dIn, int hIn, int wIn, int c) {
+    Shape sf{n, dIn, hIn, wIn};
+    const long long m = (long long)n * dIn * hIn * wIn * c;
+    hipLaunchKernelGGL(k_pool_bwd, g1(m), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, sf, c);
+    return ws_check("pool backward");
+}
+
+int lbmnn_up_fwd(const void *X, const void *skip, void *Y, int n, int dOut, int hOut, int wOut, int c) {
+    Shape sf{n, dOut, hOut, wOut};
+    const long long m = (long long)n * dOut * hOut * wOut * c;
+    hipLaunchKernelGGL(k_up_fwd, g1(m), dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)skip, (bf16_t *)Y, sf, c);
+    return ws_check("upsample forward");
+}
+
+int lbmnn_up_bwd(const void *dY, void *dX, int n, int dIn, int hIn, int wIn, int c) {
+    Shape so{n, dIn, hIn, wIn};
+    const long long m = (long long)n * dIn * hIn * wIn * c;
+    hipLaunchKernelGGL(k_up_bwd, g1(m), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, so, c);
+    return ws_check("upsample backward");
+}
+
+int lbmnn_add_inplace(void *a, const void *b, long long n) {
+    hipLaunchKernelGGL(k_add_inplace, g1(n), dim3(256), 0, 0, (bf16_t *)a, (const bf16_t *)b, n);
+    return ws_check("add");
+}
+
+int lbmnn_mse(const float *Yf, const float *T, void *dY, float *loss, long long n) {
+    hipLaunchKernelGGL(k_mse, g1(n), dim3(256), 0, 0, Yf, T, (bf16_t *)dY, loss, n);
+    return ws_check("mse");
+}
+
+int lbmnn_adamw(float *w, void *wb, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
+                float eps, float wd, int t) {
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adamw, g1(n), dim3(256), 0, 0, w, (bf16_t *)wb, g, m, v, n, lr, b1, b2, eps, wd, c1, c2);
+    return ws_check("adamw");
+}
+
+int lbmnn_f32_to_bf16(const float *a, void *b, long long n) {
+    hipLaunchKernelGGL(k_f32_to_bf16, g1(n), dim3(256), 0, 0, a, (bf16_t *)b, n);
+    return ws_check("convert");
+}
+
+int lbmnn_memset(void *p, int value, unsigned long long bytes) {
+    return hipMemset(p, value, bytes) == hipSuccess;
+}
+
+int lbmnn_synchronize(void) { return hipDeviceSynchronize() == hipSuccess; }
+
+}  // extern "C"

@@ -7,6 +7,9 @@
       and BF16 population storage; Cd/Cl with the pressure/friction split, and
       steps/seconds until the windowed Cd settles within 1 % of the FP64 value.
 
+  python -m lattice_fluid_dynamics_amd.experiments warmstart [--train 384 --eval 32 --iters 3000]
+      Config 5: learned warm start (see warmstart_experiment).
+
   python -m lattice_fluid_dynamics_amd.experiments sweep [--tunnels 64 --steps 5000]
       Config 4: tunnels with nx ~ U[64,192], ny = nz = nx/2, a seeded random box,
       ellipsoid or slanted-rear body 10-20 % of ny, u ~ U[0.02,0.08],
@@ -68,33 +71,9 @@ def storage_experiment(steps: int, interval: int = 200):
 
 
 def sweep_bodies(seed: int, nx: int, ny: int, nz: int):
-    """Seeded random box, ellipsoid or slanted-rear body, 10-20 % of ny tall."""
-    rng = np.random.default_rng(seed)
-    kind = int(rng.integers(0, 3))
-    h = rng.uniform(0.10, 0.20) * 4.0  # world height (ny spans 4 world units)
-    L = h * rng.uniform(1.5, 2.5)
-    W = h * rng.uniform(0.8, 1.2)
-    c = np.array([-2.0 + rng.uniform(-0.3, 0.3), rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2)])
-    if kind == 0:
-        def r(d):
-            return geometry._box_radius(d, np.array([L, W, h]) / 2)
-    elif kind == 1:
-        def r(d):
-            return geometry._ellipsoid_radius(d, np.array([L, W, h]) / 2, np.zeros(3))
-    else:
-        ang = np.radians(rng.uniform(20, 35))
-        n_sl = np.array([np.sin(ang), 0.0, np.cos(ang)])
-        p0 = np.array([L / 2 - 0.4 * L, 0.0, h / 2])
-
-        def r(d):
-            box = geometry._box_radius(d, np.array([L, W, h]) / 2)
-            dn = d @ n_sl
-            with np.errstate(divide="ignore", invalid="ignore"):
-                t = np.where(dn > 1e-9, (p0 @ n_sl) / dn, np.inf)
-            return np.minimum(box, t)
-    tris = geometry.radial_mesh(r, c, 40, 40)
-    v = tris.reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
-    return tris, (*v.min(0).tolist(), *v.max(0).tolist()), ["box", "ellipsoid", "slanted-rear"][kind]
+    """Seeded random box, ellipsoid or slanted-rear body, 10-20 % of ny tall
+    (geometry.random_body; the body is given in world units)."""
+    return geometry.random_body(seed)
 
 
 def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 20):
@@ -132,17 +111,126 @@ def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 
         _emit({"experiment": "config4-tunnel", **m})
 
 
+def _steps_to_1pct(h, ref, window):
+    t = convergence.time_to_within(h, ref, 0.01, window=window)
+    return None if t is None else int(t)
+
+
+def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 6000, avg_steps: int = 2000,
+                         cold_steps: int = 20000, warm_steps: int = 12000, batch: int = 16, lr: float = 1e-3,
+                         weights_in=None, weights_out=None, chunk: int = 96, window: int = 20,
+                         dims=(256, 128, 128), factor: int = 4, phase: str = "all"):
+    """Config 5: learned warm start at 256x128x128 (coarse lattice 64x32x32).
+
+    Training tunnels use body seeds 0..n_train-1, held-out tunnels seeds
+    100000..; inlet speed U[0.03, 0.07] and tau U[0.6, 0.9] per seed (MRT,
+    periodic y/z, steady laminar wakes, Re ~ 3-50).  Targets: the FP32
+    cold-start solution after `data_steps`, block-averaged to the coarse
+    lattice and averaged over the next `avg_steps`.  Evaluation: Cd sampled
+    every 100 steps; time to converged Cd = first step after which the mean
+    of the last `window` samples stays within 1 % of the held-out tunnel's
+    FP32 cold-start converged Cd (mean of its last 6000 steps)."""
+    from . import warmstart as WS
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    nx, ny, nz = dims
+    ws = WS.WarmStart(nx // factor, ny // factor, nz // factor, max_batch=batch, seed=0)
+    if weights_in:
+        ws.load(weights_in)
+    if phase in ("all", "train") and not weights_in:
+        specs = WS.random_specs(range(n_train))
+        t0 = time.time()
+        masks, fields = WS.solve_fields(specs, dims, factor, data_steps, avg_steps, 100, storage=0, chunk=chunk,
+                                        log=log)
+        gen_s = time.time() - t0
+        data = WS.DeviceDataset(masks, fields, specs, augment=True)
+        t0 = time.time()
+        losses = WS.train(ws, data, iters, batch=batch, lr=lr, wd=1e-4, log=log)
+        train_s = time.time() - t0
+        if weights_out:
+            ws.save(weights_out)
+        _emit({"experiment": "config5-train", "train_tunnels": n_train, "samples_with_mirrors": data.n,
+               "lattice": f"{nx}x{ny}x{nz}", "coarse": f"{nx // factor}x{ny // factor}x{nz // factor}",
+               "data_steps": data_steps, "avg_steps": avg_steps, "data_gen_s": round(gen_s, 1),
+               "iters": iters, "batch": batch, "lr": lr, "train_s": round(train_s, 1),
+               "ms_per_iter": round(train_s / max(iters, 1) * 1e3, 2), "params": ws.param_count,
+               "loss_first": round(float(losses[0]), 5),
+               "loss_last100": round(float(np.mean(losses[-100:])), 5),
+               "loss_every_100": [round(float(np.mean(losses[k:k + 100])), 5) for k in range(0, len(losses), 100)]})
+        del data
+    if phase == "train":
+        return
+    specs = WS.random_specs(range(100000, 100000 + n_eval))
+    runs = {}
+    runs["cold_fp32"] = WS.cd_histories(specs, dims, cold_steps, 0, None, factor, avg_from=cold_steps - 6000, log=log)
+    runs["cold_fp16"] = WS.cd_histories(specs, dims, warm_steps, 1, None, factor, log=log)
+    runs["warm_fp16"] = WS.cd_histories(specs, dims, warm_steps, 1, ws, factor, log=log)
+    runs["warm_fp32"] = WS.cd_histories(specs, dims, warm_steps, 0, ws, factor, log=log)
+    conv = runs["cold_fp32"]["mean_field"]
+    rows = []
+    for t, sp in enumerate(specs):
+        ref = runs["cold_fp32"]["histories"][t].converged_value(tail=60)
+        row = {"seed": sp.seed, "u": round(sp.u, 5), "tau": round(sp.tau, 4), "cd_ref": round(ref, 5)}
+        for name, r in runs.items():
+            st = _steps_to_1pct(r["histories"][t], ref, window)
+            per_tunnel_ms = r["ms_per_step"] / len(specs)
+            row[name + "_steps"] = st
+            row[name + "_s"] = None if st is None else round(st * per_tunnel_ms / 1e3 + r["init_s"], 4)
+        fluid = runs["cold_fp32"]["masks"][t] < 0.5
+        pred = runs["warm_fp16"]["predicted"][t]
+        du = np.linalg.norm((pred[..., :3] - conv[t, ..., :3])[fluid]) / np.linalg.norm(conv[t, ..., :3][fluid])
+        uni = np.zeros_like(conv[t, ..., :3])
+        uni[..., 0] = sp.u
+        du0 = np.linalg.norm((uni - conv[t, ..., :3])[fluid]) / np.linalg.norm(conv[t, ..., :3][fluid])
+        row["pred_u_rel_l2"] = round(float(du), 4)
+        row["uniform_u_rel_l2"] = round(float(du0), 4)
+        rows.append(row)
+        _emit({"experiment": "config5-tunnel", **row})
+
+    def med(key):
+        v = [r[key] for r in rows if r[key] is not None]
+        return None if not v else float(np.median(v))
+
+    ratios = [r["cold_fp32_s"] / r["warm_fp16_s"] for r in rows
+              if r["cold_fp32_s"] is not None and r["warm_fp16_s"] is not None]
+    _emit({"experiment": "config5-summary", "eval_tunnels": n_eval, "window_samples": window,
+           "sample_interval": 100,
+           **{k + "_ms_per_step_per_tunnel": round(r["ms_per_step"] / n_eval, 5) for k, r in runs.items()},
+           **{k + "_converged": sum(r2[k + "_steps"] is not None for r2 in rows) for k in runs},
+           **{k + "_median_steps": med(k + "_steps") for k in runs},
+           **{k + "_median_s": med(k + "_s") for k in runs},
+           "warm_init_s_per_tunnel": round(runs["warm_fp16"]["init_s"], 4),
+           "speedup_cold_fp32_over_warm_fp16_median": None if not ratios else round(float(np.median(ratios)), 3),
+           "speedup_geomean": None if not ratios else round(float(np.exp(np.mean(np.log(ratios)))), 3),
+           "pred_u_rel_l2_median": med("pred_u_rel_l2"), "uniform_u_rel_l2_median": med("uniform_u_rel_l2")})
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["storage", "sweep"])
+    ap.add_argument("which", choices=["storage", "sweep", "warmstart"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--tunnels", type=int, default=64)
     ap.add_argument("--storage", type=int, default=1)
+    ap.add_argument("--train", type=int, default=384, help="warmstart: training tunnels")
+    ap.add_argument("--eval", type=int, default=32, help="warmstart: held-out tunnels")
+    ap.add_argument("--iters", type=int, default=3000, help="warmstart: training iterations")
+    ap.add_argument("--data-steps", type=int, default=6000)
+    ap.add_argument("--cold-steps", type=int, default=20000)
+    ap.add_argument("--warm-steps", type=int, default=12000)
+    ap.add_argument("--weights-in", default=None)
+    ap.add_argument("--weights-out", default=None)
+    ap.add_argument("--phase", choices=["all", "train", "eval"], default="all")
     a = ap.parse_args(argv)
     if a.which == "storage":
         storage_experiment(a.steps or 20000)
-    else:
+    elif a.which == "sweep":
         sweep_experiment(a.tunnels, a.steps or 5000, a.storage)
+    else:
+        warmstart_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps, cold_steps=a.cold_steps,
+                             warm_steps=a.warm_steps, weights_in=a.weights_in, weights_out=a.weights_out,
+                             phase=a.phase)
 
 
 if __name__ == "__main__":

@@ -124,3 +124,37 @@ def ahmed_body(nx: int, ny: int, nz: int, cells=(64, 24, 18), front_radius_cells
     tris = radial_mesh(radius, c, n_lat, n_lon)
     v = tris.reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
     return tris, (*v.min(0).tolist(), *v.max(0).tolist())
+
+
+def random_body(seed: int, n_lat: int = 40, n_lon: int = 40):
+    """Seeded random bluff body for parameter sweeps and warm-start data: a
+    box, an ellipsoid or a box with a slanted rear (20-35 deg), 10-20 % of
+    the tunnel height tall (the tunnel spans 8 x 4 x 4 world units), length
+    1.5-2.5 and width 0.8-1.2 times the height, centred near x = -2.
+    Returns (triangles, bbox, kind)."""
+    rng = np.random.default_rng(seed)
+    kind = int(rng.integers(0, 3))
+    h = rng.uniform(0.10, 0.20) * 4.0
+    L = h * rng.uniform(1.5, 2.5)
+    W = h * rng.uniform(0.8, 1.2)
+    c = np.array([-2.0 + rng.uniform(-0.3, 0.3), rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2)])
+    if kind == 0:
+        def r(d):
+            return _box_radius(d, np.array([L, W, h]) / 2)
+    elif kind == 1:
+        def r(d):
+            return _ellipsoid_radius(d, np.array([L, W, h]) / 2, np.zeros(3))
+    else:
+        ang = np.radians(rng.uniform(20, 35))
+        n_sl = np.array([np.sin(ang), 0.0, np.cos(ang)])
+        p0 = np.array([L / 2 - 0.4 * L, 0.0, h / 2])
+
+        def r(d):
+            box = _box_radius(d, np.array([L, W, h]) / 2)
+            dn = d @ n_sl
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = np.where(dn > 1e-9, (p0 @ n_sl) / dn, np.inf)
+            return np.minimum(box, t)
+    tris = radial_mesh(r, c, n_lat, n_lon)
+    v = tris.reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
+    return tris, (*v.min(0).tolist(), *v.max(0).tolist()), ["box", "ellipsoid", "slanted-rear"][kind]

@@ -57,6 +57,7 @@ SIGNATURES = {
     "LBM_Free": (None, [GridP]),
     "LBM_SetSolidAABB": (None, [GridP] + [c_float] * 6),
     "LBM_InitializeFlow": (c_int, [GridP, c_float, c_float, c_float]),
+    "LBM_InitializeFlowFromField": (c_int, [GridP, c_float_p, c_int, c_int, c_int]),
     "LBM_Step": (None, [GridP, c_float, c_float, c_float]),
     "LBM_GetVelocityBuffer": (c_void_p, [GridP]),
     "LBM_ComputeDragForce": (None, [GridP, c_float_p, c_float_p, c_float_p]),
@@ -81,6 +82,7 @@ SIGNATURES = {
     "LBM_DeviceFree": (None, [c_void_p]),
     "LBM_CopyToDevice": (c_int, [c_void_p, c_void_p, ctypes.c_ulonglong]),
     "LBM_CopyToHost": (c_int, [c_void_p, c_void_p, ctypes.c_ulonglong]),
+    "LBM_CopyDeviceToDevice": (c_int, [c_void_p, c_void_p, ctypes.c_ulonglong]),
     "LBM_Synchronize": (None, [GridP]),
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
@@ -108,6 +110,9 @@ SIGNATURES = {
     "LBM_BatchComputeCoefficients": (c_int, [c_void_p, c_float_p]),
     "LBM_BatchCopyPopulationsToHostF64": (c_int, [c_void_p, c_int, c_double_p, c_double_p]),
     "LBM_BatchCopyVelocityToHost": (c_int, [c_void_p, c_int, c_float_p]),
+    "LBM_BatchInitializeTunnelFromField": (c_int, [c_void_p, c_int, c_float_p, c_int, c_int, c_int]),
+    "LBM_BatchCopyCoarseVelocityToHost": (c_int, [c_void_p, c_int, c_int, c_float_p]),
+    "LBM_CopyCoarseVelocityToHost": (c_int, [GridP, c_int, c_float_p]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -209,6 +214,15 @@ def LBM_InitializeFlow(g: Grid, ux: float, uy: float, uz: float) -> int:
     return int(load_library().LBM_InitializeFlow(g.ptr, ux, uy, uz))
 
 
+def LBM_InitializeFlowFromField(g: Grid, field: np.ndarray) -> int:
+    """Equilibrium initial state from a coarse (cnz, cny, cnx, 4) field of
+    (ux, uy, uz, rho), trilinearly upsampled on the device."""
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    assert f.ndim == 4 and f.shape[3] == 4
+    cnz, cny, cnx = f.shape[:3]
+    return load_library().LBM_InitializeFlowFromField(g.ptr, _fp(f), cnx, cny, cnz)
+
+
 def LBM_Step(g: Grid, ux: float, uy: float, uz: float) -> None:
     load_library().LBM_Step(g.ptr, ux, uy, uz)
 
@@ -246,6 +260,15 @@ def LBM_ComputeLiftCoefficient(g: Grid, U: float, A: float) -> float:
 def LBM_CopyVelocityToHost(g: Grid) -> np.ndarray:
     out = np.empty((g.totalCells, 4), np.float32)
     if not load_library().LBM_CopyVelocityToHost(g.ptr, _fp(out)):
+        raise RuntimeError(last_error())
+    return out
+
+
+def LBM_CopyCoarseVelocityToHost(g: Grid, factor: int) -> np.ndarray:
+    """Block means of (ux, uy, uz, rho) over factor^3 cells, shape (nz/f, ny/f, nx/f, 4)."""
+    nx, ny, nz = g.shape
+    out = np.empty((nz // factor, ny // factor, nx // factor, 4), np.float32)
+    if not load_library().LBM_CopyCoarseVelocityToHost(g.ptr, factor, _fp(out)):
         raise RuntimeError(last_error())
     return out
 
@@ -434,6 +457,14 @@ class Batch:
             raise RuntimeError(last_error())
         return int(bs), ms
 
+    def initialize_from_field(self, tunnel: int, field: np.ndarray):
+        """Replaces the tunnel's uniform start with the equilibrium of a coarse
+        (cnz, cny, cnx, 4) (ux, uy, uz, rho) field (before the first step)."""
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        cnz, cny, cnx = f.shape[:3]
+        if not load_library().LBM_BatchInitializeTunnelFromField(self.ptr, tunnel, _fp(f), cnx, cny, cnz):
+            raise RuntimeError("LBM_BatchInitializeTunnelFromField failed")
+
     def set_block_size(self, bs: int):
         if not load_library().LBM_BatchSetBlockSize(self.ptr, bs):
             raise RuntimeError("invalid block size")
@@ -473,6 +504,15 @@ class Batch:
         out = np.empty((cells, 4), np.float32)
         if not load_library().LBM_BatchCopyVelocityToHost(self.ptr, tunnel, _fp(out)):
             raise RuntimeError(last_error())
+        return out
+
+    def coarse_velocity(self, tunnel: int, dims, factor: int) -> np.ndarray:
+        """Block means of (ux, uy, uz, rho) over factor^3 cells of one tunnel
+        (dims = its (nx, ny, nz)), shape (nz/f, ny/f, nx/f, 4)."""
+        nx, ny, nz = dims
+        out = np.empty((nz // factor, ny // factor, nx // factor, 4), np.float32)
+        if not load_library().LBM_BatchCopyCoarseVelocityToHost(self.ptr, tunnel, factor, _fp(out)):
+            raise RuntimeError("LBM_BatchCopyCoarseVelocityToHost failed")
         return out
 
     def __del__(self):

@@ -1,5 +1,6 @@
-"""The C ABI: liblbm_amd.so loads and exports every function include/lbm.h
-declares, and the Python mirror of LBMGrid has the C layout.  No GPU calls."""
+"""The C ABI: liblbm_amd.so loads and exports every function include/*.h
+declares, the Python bindings cover exactly those, and the Python mirror of
+LBMGrid has the C layout.  No GPU calls."""
 import ctypes
 import os
 import re
@@ -8,20 +9,27 @@ import tempfile
 
 import pytest
 
-from lattice_fluid_dynamics_amd import lbm
+from lattice_fluid_dynamics_amd import lbm, warmstart
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "lbm.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_warmstart.h")]
 
 
-def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(LBM_[A-Za-z0-9_]+)\s*\(", text)))
+def declared_functions(headers=HEADERS):
+    names = set()
+    for h in headers:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b((?:LBM|lbmnn)_[A-Za-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_header_is_covered():
+    assert sorted(HEADERS) == sorted(os.path.join(ROOT, "include", h)
+                                     for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h"))
 
 
 def test_header_declares_the_reference_api():
-    names = set(declared_functions())
+    names = set(declared_functions(HEADERS[:1]))
     reference_api = {  # simulation/lib/lbm.h:58-129
         "LBM_Create", "LBM_Free", "LBM_SetSolidAABB", "LBM_InitializeFlow", "LBM_Step",
         "LBM_GetVelocityBuffer", "LBM_ComputeDragForce", "LBM_ComputeDragForceDecomposed",
@@ -31,11 +39,11 @@ def test_header_declares_the_reference_api():
 
 
 def test_library_exports_every_declared_symbol():
-    L = lbm.load_library()
+    L = warmstart.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert set(declared_functions()) == set(lbm.SIGNATURES), \
-        set(declared_functions()) ^ set(lbm.SIGNATURES)
+    bound = set(lbm.SIGNATURES) | set(warmstart.WS_SIGNATURES)
+    assert set(declared_functions()) == bound, set(declared_functions()) ^ bound
 
 
 def test_grid_struct_layout_matches_c():
