@@ -184,6 +184,8 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
         uni = np.zeros_like(conv[t, ..., :3])
         uni[..., 0] = sp.u
         du0 = np.linalg.norm((uni - conv[t, ..., :3])[fluid]) / np.linalg.norm(conv[t, ..., :3][fluid])
+        for name in ("cold_fp32", "warm_fp16"):
+            row[name + "_cd_first_30"] = [round(v, 4) for v in runs[name]["histories"][t].cd[:30]]
         row["pred_u_rel_l2"] = round(float(du), 4)
         row["uniform_u_rel_l2"] = round(float(du0), 4)
         rows.append(row)

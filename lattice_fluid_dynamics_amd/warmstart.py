@@ -236,6 +236,45 @@ class WarmStart:
             pass
 
 
+# network layout of include/lbm_warmstart.h (convolution widths and levels)
+CONV_CIN = [32, 32, 32, 64, 64, 128, 128, 256, 256, 128, 128, 64, 64, 32, 32]
+CONV_COUT = [32, 32, 64, 64, 128, 128, 256, 256, 128, 128, 64, 64, 32, 32, 32]
+CONV_LEVEL = [0, 0, 1, 1, 2, 2, 3, 3, 3, 2, 2, 1, 1, 0, 0]
+
+
+def flops_per_sample(cshape, training: bool = True) -> float:
+    """Matrix-core FLOP of one sample: 2 * positions * 27 * cin * cout per
+    convolution; training adds the input gradient (all but the first
+    convolution) and the weight gradient."""
+    cz, cy, cx = cshape
+    total = 0.0
+    for k, (ci, co, lv) in enumerate(zip(CONV_CIN, CONV_COUT, CONV_LEVEL)):
+        pos = (cz >> (lv + 1)) * (cy >> (lv + 1)) * (cx >> (lv + 1))
+        f = 2.0 * pos * 27 * ci * co
+        total += f
+        if training:
+            total += f + (f if k > 0 else 0.0)
+    return total
+
+
+def train_benchmark(cshape=(32, 32, 64), batch: int = 16, iters: int = 50, seed: int = 0):
+    """Training steps on random packed data: ms per step and matrix-core TFLOP/s."""
+    cz, cy, cx = cshape
+    ws = WarmStart(cx, cy, cz, max_batch=batch, seed=seed)
+    rng = np.random.default_rng(seed)
+    P = ws.packed_positions
+    x = DeviceArray.from_host(np.where(rng.uniform(size=(batch, P, 32)) > 0.5, 0x3F80, 0).astype(np.uint16))
+    t = DeviceArray.from_host(rng.normal(size=(batch, P, 32)).astype(np.float32))
+    for _ in range(3):
+        ws.train_step(x, t, batch, 1e-4, 0.0)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        ws.train_step(x, t, batch, 1e-4, 0.0)
+    dt = (time.perf_counter() - t0) / iters
+    fl = flops_per_sample(cshape) * batch
+    return {"ms_per_step": dt * 1e3, "tflops": fl / dt / 1e12, "gflop_per_step": fl / 1e9}
+
+
 # ------------------------------------------------------------ data
 @dataclass
 class TunnelSpec:
