@@ -88,7 +88,9 @@ int LBM_WarmStartInitialize(LBMGrid *grid, LBMWarmStart *ws, float u);
 int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, float *Yf, int n, int d, int h, int w,
                    int cin, int cout, int relu);
 int lbmnn_conv_mirror(const void *W, void *Wm, int cin, int cout);
-int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, int n, int d, int h, int w, int cin, int cout);
+/* dW += weight gradient and (db != NULL) db += bias gradient; cin, cout % 32 == 0 */
+int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n, int d, int h, int w, int cin,
+                     int cout);
 int lbmnn_relu_bwd(void *dY, const void *Y, long long n);
 int lbmnn_bias_grad(const void *dZ, float *db, long long P, int c);
 int lbmnn_pool_fwd(const void *X, void *Y, int n, int dOut, int hOut, int wOut, int c);

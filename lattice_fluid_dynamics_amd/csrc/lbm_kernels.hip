@@ -597,36 +597,61 @@ struct BoundArgs {
 };
 
 // Boundary prepass: the incoming populations of every listed cell under the
-// reference stream rule, evaluated on the stored post-collision field.
+// reference stream rule, evaluated on the stored post-collision field.  One
+// lane per (listed cell, direction): the list is padded to a multiple of 64
+// per direction, so each wavefront evaluates a single direction (made
+// explicit with readfirstlane and dispatched to a constant-direction copy of
+// the rule), and stores to ovr_vals[i * n + k] are contiguous.
+#define LBM_PREPASS_CASE(I) \
+    case I: v = stream_value<FMT, STATE>(pop, g, periodic, x, y, z, I, qo); break;
+
+template <int FMT, int STATE>
+__device__ __forceinline__ typename Pop<FMT>::T prepass_value(const typename Pop<FMT>::S *pop, const Geo &g,
+                                                              int periodic, int x, int y, int z, int i, float qo) {
+    typename Pop<FMT>::T v = 0;
+    switch (i) {
+        LBM_PREPASS_CASE(0) LBM_PREPASS_CASE(1) LBM_PREPASS_CASE(2) LBM_PREPASS_CASE(3) LBM_PREPASS_CASE(4)
+        LBM_PREPASS_CASE(5) LBM_PREPASS_CASE(6) LBM_PREPASS_CASE(7) LBM_PREPASS_CASE(8) LBM_PREPASS_CASE(9)
+        LBM_PREPASS_CASE(10) LBM_PREPASS_CASE(11) LBM_PREPASS_CASE(12) LBM_PREPASS_CASE(13)
+        LBM_PREPASS_CASE(14) LBM_PREPASS_CASE(15) LBM_PREPASS_CASE(16) LBM_PREPASS_CASE(17)
+        LBM_PREPASS_CASE(18)
+    }
+    return v;
+}
+#undef LBM_PREPASS_CASE
+
+__host__ __device__ inline int prepass_pad(int n) { return (n + 63) & ~63; }
+
 template <int FMT, int STATE>
 __global__ __launch_bounds__(BLOCK) void k_prepass(BoundArgs<FMT> a) {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= a.n_ovr) return;
+    if (a.n_ovr <= 0) return;
+    const int npad = prepass_pad(a.n_ovr);
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(idx / npad);  // uniform: npad and waves are 64-aligned
+    const int k = idx - i * npad;
+    if (i >= 19 || k >= a.n_ovr) return;
     int x, y, z;
     decompose(a.g, a.ovr_cells[k], x, y, z);
-#pragma unroll
-    for (int i = 0; i < 19; i++) {
-        const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
-        a.ovr_vals[(long long)i * a.n_ovr + k] =
-            stream_value<FMT, STATE>(a.pop, a.g, a.periodic, x, y, z, i, qo);
-    }
+    const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
+    a.ovr_vals[(long long)i * a.n_ovr + k] = prepass_value<FMT, STATE>(a.pop, a.g, a.periodic, x, y, z, i, qo);
 }
 
 // Batched boundary prepass: list entry k belongs to tunnel ovr_tunnel[k].
 template <int FMT, int STATE>
 __global__ __launch_bounds__(BLOCK) void k_prepass_batch(BoundArgs<FMT> a, const int32_t *ovr_tunnel,
                                                          const TunnelDesc *desc) {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= a.n_ovr) return;
+    if (a.n_ovr <= 0) return;
+    const int npad = prepass_pad(a.n_ovr);
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(idx / npad);
+    const int k = idx - i * npad;
+    if (i >= 19 || k >= a.n_ovr) return;
     const TunnelDesc d = desc[ovr_tunnel[k]];
-    const typename Pop<FMT>::S *pop = a.pop + d.popOff;
     int x, y, z;
     decompose(d.g, a.ovr_cells[k], x, y, z);
-#pragma unroll
-    for (int i = 0; i < 19; i++) {
-        const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
-        a.ovr_vals[(long long)i * a.n_ovr + k] = stream_value<FMT, STATE>(pop, d.g, a.periodic, x, y, z, i, qo);
-    }
+    const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
+    a.ovr_vals[(long long)i * a.n_ovr + k] =
+        prepass_value<FMT, STATE>(a.pop + d.popOff, d.g, a.periodic, x, y, z, i, qo);
 }
 
 // ---------------------------------------------------------------- force
@@ -1233,7 +1258,7 @@ static void launch_prepass(LbmDev *d, int periodic) {
     a.n_ovr = d->n_ovr;
     a.periodic = periodic;
     a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
-    const int nb = (d->n_ovr + BLOCK - 1) / BLOCK;
+    const int nb = (19 * prepass_pad(d->n_ovr) + BLOCK - 1) / BLOCK;
     if (d->state == ST_A)
         hipLaunchKernelGGL((k_prepass<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
     else
@@ -1959,7 +1984,7 @@ static void batch_step_once(LbmBatchDev *b, const LbmBatchParams *p, int wv) {
             a.n_ovr = b->n_ovr;
             a.periodic = p->periodicYZ == 1;
             a.g = make_geo(2, 1, 1, 64);
-            const int nb = (b->n_ovr + BLOCK - 1) / BLOCK;
+            const int nb = (19 * prepass_pad(b->n_ovr) + BLOCK - 1) / BLOCK;
             if (b->state == ST_A)
                 hipLaunchKernelGGL((k_prepass_batch<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, b->stream, a,
                                    (const int32_t *)b->ovr_tunnel, (const TunnelDesc *)b->desc_d);

@@ -238,8 +238,7 @@ static int conv_bwd(LBMWarmStart *ws, int k, void *gout, const void *in, void *g
     Conv *c = &ws->conv[k];
     int ok = 1;
     if (relu) ok = ok && lbmnn_relu_bwd(gout, out, P * COUT[k]);
-    ok = ok && lbmnn_conv_wgrad(gout, in, c->gw, B, d[0], d[1], d[2], CIN[k], COUT[k]);
-    ok = ok && lbmnn_bias_grad(gout, c->gb, P, COUT[k]);
+    ok = ok && lbmnn_conv_wgrad(gout, in, c->gw, c->gb, B, d[0], d[1], d[2], CIN[k], COUT[k]);
     if (gin) ok = ok && lbmnn_conv_fwd(gout, c->wm, NULL, gin, NULL, B, d[0], d[1], d[2], COUT[k], CIN[k], 0);
     return ok;
 }

@@ -135,45 +135,92 @@ __global__ void k_conv_mirror(const bf16_t *W, bf16_t *Wm, int cin, int cout) {
 }
 
 // ---------------------------------------------------------------- wgrad
-// dW[co][t][ci] += sum_p dZ[p][co] * X[p + off(t)][ci] over a chunk of
-// positions per workgroup (K = positions), FP32 atomics into dW.
-// Workgroup: 4 waves, each a 16x16 (co x ci) block of one tap; grid
-// (position chunks, 27 taps, co/16 * ci/16 blocks / 4).
+// dW[co][t][ci] += sum_p dZ[p][co] * X[p + off(t)][ci]: a GEMM with the
+// positions as K.  A workgroup owns a TCO x TCI block of one tap and a chunk
+// of positions; per slab of 32 positions it loads the dZ and shifted X tiles
+// with 16-byte loads (one position's 8 consecutive channels per lane),
+// stores them transposed into LDS (channel-major, positions contiguous), and
+// every wave reads its MFMA operands back as 16-byte LDS loads.  Partial
+// sums go to dW with FP32 atomics.  Workgroups of the centre tap with the
+// first ci block also accumulate the bias gradient sum_p dZ[p][co] into db.
+constexpr int WG_K = 32;       // positions per slab
+constexpr int WG_LD = WG_K + 8; // LDS row length (bf16), keeps rows 16-byte aligned
+
+template <int TCO, int TCI>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t *__restrict__ dZ, const bf16_t *__restrict__ X,
-                                                    float *__restrict__ dW, Shape s, int cin, int cout,
-                                                    int chunk) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                                    float *__restrict__ dW, float *__restrict__ db, Shape s,
+                                                    int cin, int cout, int chunk) {
+    __shared__ __align__(16) bf16_t Ds[TCO * WG_LD];
+    __shared__ __align__(16) bf16_t Xs[TCI * WG_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int P = s.n * s.d * s.h * s.w;
     const int t = blockIdx.y;
-    const int nbci = cin / 16;
-    const int blk = blockIdx.z * 4 + wave;
-    if (blk >= (cout / 16) * nbci) return;
-    const int co0 = (blk / nbci) * 16, ci0 = (blk % nbci) * 16;
-    const int r = lane & 15, kq = lane >> 4;
+    const int nbci = cin / TCI;
+    const int co0 = (blockIdx.z / nbci) * TCO, ci0 = (blockIdx.z % nbci) * TCI;
     const int pbeg = blockIdx.x * chunk, pend = min(P, pbeg + chunk);
-    f32x4 acc = f32x4{0, 0, 0, 0};
-    for (int pk = pbeg; pk < pend; pk += 32) {
-        bf16x8 a, b;
+    const bool do_bias = db != nullptr && t == 13 && ci0 == 0;
+    constexpr int NBLK = (TCO / 16) * (TCI / 16);
+    constexpr int PER_WAVE = (NBLK + 3) / 4;
+    const int r = lane & 15, kq = lane >> 4;
+    f32x4 acc[PER_WAVE];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int p = pk + 8 * kq + j;
-            bf16_t av = 0, bv = 0;
-            if (p < pend) {
-                av = dZ[(long long)p * cout + co0 + r];
-                int q;
-                if (tap_pos(s, p, t, q)) bv = X[(long long)q * cin + ci0 + r];
-            }
-            a[j] = __builtin_bit_cast(__bf16, av);
-            b[j] = __builtin_bit_cast(__bf16, bv);
+    for (int b = 0; b < PER_WAVE; b++) acc[b] = f32x4{0, 0, 0, 0};
+    float bsum = 0.0f;
+    for (int p0 = pbeg; p0 < pend; p0 += WG_K) {
+        // dZ tile: WG_K positions x TCO channels, TCO/8 lanes per position
+        for (int e = tid; e < WG_K * TCO / 8; e += 256) {
+            const int pl = e / (TCO / 8), cg = e % (TCO / 8);
+            const int p = p0 + pl;
+            Pack8 v;
+            v.u = make_uint4(0, 0, 0, 0);
+            if (p < pend) v.u = *(const uint4 *)(dZ + (long long)p * cout + co0 + 8 * cg);
+            const bf16_t *h = (const bf16_t *)&v.u;
+#pragma unroll
+            for (int j = 0; j < 8; j++) Ds[(8 * cg + j) * WG_LD + pl] = h[j];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-    }
-    // D[row = co][col = ci]
+        for (int e = tid; e < WG_K * TCI / 8; e += 256) {
+            const int pl = e / (TCI / 8), cg = e % (TCI / 8);
+            const int p = p0 + pl;
+            Pack8 v;
+            v.u = make_uint4(0, 0, 0, 0);
+            int q;
+            if (p < pend && tap_pos(s, p, t, q)) v.u = *(const uint4 *)(X + (long long)q * cin + ci0 + 8 * cg);
+            const bf16_t *h = (const bf16_t *)&v.u;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int co = co0 + 4 * kq + j, ci = ci0 + r;
-        atomicAdd(&dW[((long long)co * 27 + t) * cin + ci], acc[j]);
+            for (int j = 0; j < 8; j++) Xs[(8 * cg + j) * WG_LD + pl] = h[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < PER_WAVE; b++) {
+            const int blk = wave + 4 * b;
+            if (blk < NBLK) {
+                const int bco = (blk / (TCI / 16)) * 16, bci = (blk % (TCI / 16)) * 16;
+                Pack8 a, bb;
+                a.u = *(const uint4 *)(Ds + (bco + r) * WG_LD + 8 * kq);
+                bb.u = *(const uint4 *)(Xs + (bci + r) * WG_LD + 8 * kq);
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bb.v, acc[b], 0, 0, 0);
+            }
+        }
+        if (do_bias && tid < TCO) {
+#pragma unroll 8
+            for (int k = 0; k < WG_K; k++) bsum += bf2f(Ds[tid * WG_LD + k]);
+        }
+        __syncthreads();
     }
+    // D[row = co][col = ci]: lane holds rows 4 kq + j of column r
+#pragma unroll
+    for (int b = 0; b < PER_WAVE; b++) {
+        const int blk = wave + 4 * b;
+        if (blk < NBLK) {
+            const int bco = (blk / (TCI / 16)) * 16, bci = (blk % (TCI / 16)) * 16;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int co = co0 + bco + 4 * kq + j, ci = ci0 + bci + r;
+                atomicAdd(&dW[((long long)co * 27 + t) * cin + ci], acc[b][j]);
+            }
+        }
+    }
+    if (do_bias && tid < TCO) atomicAdd(&db[co0 + tid], bsum);
 }
 
 // ------------------------------------------------------------ elementwise
@@ -280,17 +327,21 @@ __global__ void k_add_inplace(bf16_t *a, const bf16_t *b, long long n) {
     a[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
 }
 
-// MSE loss on FP32 predictions: dY = 2 (Y - T) / n (BF16), loss partials
-__global__ void k_mse(const float *Yf, const float *T, bf16_t *dY, float *loss, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    float l = 0;
-    if (i < n) {
+// MSE loss on FP32 predictions: dY = 2 (Y - T) / n (BF16); the loss is
+// reduced per workgroup (wave shuffles, then LDS) into one atomic add.
+__global__ __launch_bounds__(256) void k_mse(const float *Yf, const float *T, bf16_t *dY, float *loss, long long n) {
+    __shared__ float part[4];
+    float l = 0.0f;
+    const float inv = 1.0f / (float)n;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
         const float d = Yf[i] - T[i];
-        dY[i] = f2bf(2.0f * d / (float)n);
-        l = d * d / (float)n;
+        dY[i] = f2bf(2.0f * d * inv);
+        l += d * d * inv;
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
 }
 
 // AdamW on FP32 master weights, refreshing the BF16 copy (decoupled decay).
@@ -358,16 +409,31 @@ int lbmnn_conv_mirror(const void *W, void *Wm, int cin, int cout) {
     return ws_check("weight mirror");
 }
 
-// dW (FP32, zeroed by the caller) += weight gradient; cin, cout % 16 == 0.
-int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, int n, int d, int h, int w, int cin, int cout) {
-    if (cin % 16 != 0 || cout % 16 != 0) return 0;
+// dW (FP32, zeroed by the caller) += weight gradient; db (may be NULL,
+// zeroed by the caller) += bias gradient.  cin, cout % 32 == 0.
+int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n, int d, int h, int w, int cin,
+                     int cout) {
+    if (cin % 32 != 0 || cout % 32 != 0) return 0;
     Shape s{n, d, h, w};
     const long long P = (long long)n * d * h * w;
-    const int chunk = 1024;
-    const int blocks = (cout / 16) * (cin / 16);
-    dim3 grid((unsigned)((P + chunk - 1) / chunk), 27, (unsigned)((blocks + 3) / 4));
-    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, 0, (const bf16_t *)dZ, (const bf16_t *)X, dW, s, cin, cout,
-                       chunk);
+    const int tco = cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
+    const int zb = (cout / tco) * (cin / tci);
+    // about 2048 workgroups, chunks a multiple of the 32-position slab
+    long long chunks = (2048 + 27LL * zb - 1) / (27LL * zb);
+    long long chunk = (P + chunks - 1) / chunks;
+    chunk = ((chunk + WG_K - 1) / WG_K) * WG_K;
+    if (chunk < WG_K) chunk = WG_K;
+    chunks = (P + chunk - 1) / chunk;
+    dim3 grid((unsigned)chunks, 27, (unsigned)zb);
+    const bf16_t *z = (const bf16_t *)dZ, *x = (const bf16_t *)X;
+    if (tco == 64 && tci == 64)
+        hipLaunchKernelGGL((k_conv_wgrad<64, 64>), grid, dim3(256), 0, 0, z, x, dW, db, s, cin, cout, (int)chunk);
+    else if (tco == 64)
+        hipLaunchKernelGGL((k_conv_wgrad<64, 32>), grid, dim3(256), 0, 0, z, x, dW, db, s, cin, cout, (int)chunk);
+    else if (tci == 64)
+        hipLaunchKernelGGL((k_conv_wgrad<32, 64>), grid, dim3(256), 0, 0, z, x, dW, db, s, cin, cout, (int)chunk);
+    else
+        hipLaunchKernelGGL((k_conv_wgrad<32, 32>), grid, dim3(256), 0, 0, z, x, dW, db, s, cin, cout, (int)chunk);
     return ws_check("conv wgrad");
 }
 
@@ -416,7 +482,8 @@ int lbmnn_add_inplace(void *a, const void *b, long long n) {
 }
 
 int lbmnn_mse(const float *Yf, const float *T, void *dY, float *loss, long long n) {
-    hipLaunchKernelGGL(k_mse, g1(n), dim3(256), 0, 0, Yf, T, (bf16_t *)dY, loss, n);
+    const long long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_mse, dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(256), 0, 0, Yf, T, (bf16_t *)dY, loss, n);
     return ws_check("mse");
 }
 

@@ -47,7 +47,7 @@ WS_SIGNATURES = {
     "LBM_WarmStartInitialize": (c_int, [lbm.GridP, c_void_p, c_float]),
     "lbmnn_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 7),
     "lbmnn_conv_mirror": (c_int, [c_void_p, c_void_p, c_int, c_int]),
-    "lbmnn_conv_wgrad": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 6),
+    "lbmnn_conv_wgrad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 6),
     "lbmnn_relu_bwd": (c_int, [c_void_p, c_void_p, ctypes.c_longlong]),
     "lbmnn_bias_grad": (c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_int]),
     "lbmnn_pool_fwd": (c_int, [c_void_p, c_void_p] + [c_int] * 5),

@@ -38,6 +38,13 @@ SHAPES = [  # (n, d, h, w, cin, cout): tails of the 128-position tiles, 1..4 col
     (1, 2, 2, 4, 256, 128),
     (3, 1, 2, 3, 32, 16),
 ]
+WGRAD_SHAPES = [  # (n, d, h, w, cin, cout): every 32/64 channel tiling, slab tails
+    (2, 3, 5, 7, 32, 32),
+    (1, 4, 4, 8, 64, 64),
+    (1, 2, 2, 4, 256, 128),
+    (3, 5, 6, 7, 32, 64),
+    (2, 8, 8, 8, 64, 32),
+]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -67,16 +74,17 @@ def test_conv_forward_and_dgrad(gpu, shape):
         assert rel_l2(DXf.download(np.float32, (n, d, h, w, cin)), C.conv_dgrad(Gv, Wv)) < 1e-6
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("shape", WGRAD_SHAPES)
 def test_conv_weight_gradient(gpu, shape):
     n, d, h, w, cin, cout = shape
     L = W.lib()
     rng = np.random.default_rng(100 + sum(shape))
     X, Xv = dev_bf16(rng.normal(size=(n, d, h, w, cin)))
     G, Gv = dev_bf16(rng.normal(size=(n, d, h, w, cout)))
-    dW = dev_zeros(cout * 27 * cin, F32)
-    assert L.lbmnn_conv_wgrad(G.ptr, X.ptr, dW.ptr, n, d, h, w, cin, cout)
+    dW, db = dev_zeros(cout * 27 * cin, F32), dev_zeros(cout, F32)
+    assert L.lbmnn_conv_wgrad(G.ptr, X.ptr, dW.ptr, db.ptr, n, d, h, w, cin, cout)
     assert rel_l2(dW.download(np.float32, (cout, 27, cin)), C.conv_wgrad(Gv, Xv)) < 1e-6
+    assert rel_l2(db.download(np.float32, cout), Gv.reshape(-1, cout).sum(0)) < 1e-6
 
 
 def test_pool_upsample_relu_bias_mse(gpu):
