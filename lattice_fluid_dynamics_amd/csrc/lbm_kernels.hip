@@ -161,6 +161,51 @@ __device__ __forceinline__ S &at(S *base, uint32_t idx) {
     return *(S *)((char *)base + idx * (uint32_t)sizeof(S));
 }
 
+// Slots of one lattice as (plane, cell).  32- and 16-bit storage go through
+// one buffer resource per plane: the plane base is scalar and the cell's byte
+// offset a 32-bit per-lane value (buffer_load/store ... offen), a form the
+// compiler cannot fold into per-lane 64-bit addresses kept live across the
+// collision.  FP64 storage uses ordinary global addressing.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base) {
+    // gfx9-family resource: stride 0, no range limit, raw 32-bit data format
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, float) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ uint16_t buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint16_t) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint16_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, 0);
+}
+
+template <typename S>
+struct SlotIO {
+    S *pop;
+    long long stride;
+    __device__ __forceinline__ SlotIO(S *p, const Geo &g) : pop(p), stride(g.stride) {}
+    __device__ __forceinline__ S ld(int i, uint32_t cell) const {
+        return buf_ld(plane_rsrc(pop + (long long)i * stride), cell * (uint32_t)sizeof(S), S());
+    }
+    __device__ __forceinline__ void st(int i, uint32_t cell, S v) const {
+        buf_st(plane_rsrc(pop + (long long)i * stride), cell * (uint32_t)sizeof(S), v);
+    }
+};
+template <>
+struct SlotIO<double> {
+    double *pop;
+    long long stride;
+    __device__ __forceinline__ SlotIO(double *p, const Geo &g) : pop(p), stride(g.stride) {}
+    __device__ __forceinline__ double ld(int i, uint32_t cell) const { return at(plane(pop, Geo{0, 0, 0, 0, stride}, i), cell); }
+    __device__ __forceinline__ void st(int i, uint32_t cell, double v) const {
+        at(pop + (long long)i * stride, cell) = v;
+    }
+};
+
 __device__ __forceinline__ int cidx(const Geo &g, int x, int y, int z) {
     return x + g.nx * (y + g.ny * z);
 }
@@ -479,6 +524,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
 
     const uint8_t m = a.mask[cell];
     const bool ovr = a.use_ovr && (m & LBMDEV_MASK_OVR);
+    const SlotIO<typename P::S> io(a.pop, g);
     T f[19];
     if (ovr) {
         const int k = a.ovr_index[cell];
@@ -486,13 +532,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
     } else if (PHASE == 0) {
 #pragma unroll
-        for (int i = 0; i < 19; i++) f[i] = P::dec(at(plane(a.pop, g, i), (uint32_t)cell), i);
+        for (int i = 0; i < 19; i++) f[i] = P::dec(io.ld(i, (uint32_t)cell), i);
     } else {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const int j = OPP[i];
             const uint32_t src = (uint32_t)(X[1 - EX[i]] + Y[1 - EY[i]] + Z[1 - EZ[i]]);
-            f[i] = P::dec(at(plane(a.pop, g, j), src), j);
+            f[i] = P::dec(io.ld(j, src), j);
         }
     }
 
@@ -512,13 +558,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const int j = OPP[i];
-            at(plane(a.pop, g, j), (uint32_t)cell) = P::enc(fo[i], j);
+            io.st(j, (uint32_t)cell, P::enc(fo[i], j));
         }
     } else {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const uint32_t dst = (uint32_t)(X[1 + EX[i]] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
-            at(plane(a.pop, g, i), dst) = P::enc(fo[i], i);
+            io.st(i, dst, P::enc(fo[i], i));
         }
     }
 }

@@ -58,24 +58,29 @@ union Pack8 {
 // ---------------------------------------------------------------- forward
 // Y[p][co] = act(bias[co] + sum_{t, ci} W[co][t][ci] * X[p + off(t)][ci])
 // Workgroup: 4 waves x 32 positions = 128 positions; NB column blocks of 16
-// output channels (blockIdx.y selects the 16*NB-channel tile).
+// output channels (blockIdx.y selects the 16*NB-channel tile).  With
+// part != NULL the 27 taps are split over gridDim.z workgroup slices
+// (split-K for the small deep levels): slice z writes its FP32 partial sum to
+// part[z][p][co] and k_conv_epilogue finishes the result.
 // act: 0 none, 1 ReLU.  Yf (optional) receives the FP32 result.
 template <int NB>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
                                                   const float *__restrict__ bias, bf16_t *__restrict__ Y,
-                                                  float *__restrict__ Yf, Shape s, int cin, int cout, int act) {
+                                                  float *__restrict__ Yf, float *__restrict__ part, Shape s,
+                                                  int cin, int cout, int act) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int P = s.n * s.d * s.h * s.w;
     const int p0 = (blockIdx.x * 4 + wave) * 32;
     const int co0 = blockIdx.y * 16 * NB;
     const int r = lane & 15, kq = lane >> 4;  // row/column within a block, k quarter
+    const int tbeg = blockIdx.z * 27 / gridDim.z, tend = (blockIdx.z + 1) * 27 / gridDim.z;
     f32x4 acc[2][NB];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[a][b] = f32x4{0, 0, 0, 0};
     const int prow[2] = {p0 + r, p0 + 16 + r};
-    for (int t = 0; t < 27; t++) {
+    for (int t = tbeg; t < tend; t++) {
         int q[2];
         bool ok[2];
 #pragma unroll
@@ -114,12 +119,51 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, 
             for (int j = 0; j < 4; j++) {
                 const int p = p0 + 16 * a + 4 * kq + j;
                 if (p >= P) continue;
+                if (part) {
+                    part[((long long)blockIdx.z * P + p) * cout + co] = acc[a][b][j];
+                    continue;
+                }
                 float v = acc[a][b][j] + bv;
                 if (act == 1) v = fmaxf(v, 0.0f);
                 Y[(long long)p * cout + co] = f2bf(v);
                 if (Yf) Yf[(long long)p * cout + co] = v;
             }
         }
+}
+
+// Sum of the split-K partials in slice order (deterministic), bias, activation.
+__global__ void k_conv_epilogue(const float *__restrict__ part, int nsplit, const float *__restrict__ bias,
+                                bf16_t *__restrict__ Y, float *__restrict__ Yf, long long n, int cout, int act) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = 0.0f;
+    for (int z = 0; z < nsplit; z++) v += part[(long long)z * n + i];
+    if (bias) v += bias[i % cout];
+    if (act == 1) v = fmaxf(v, 0.0f);
+    Y[i] = f2bf(v);
+    if (Yf) Yf[i] = v;
+}
+
+// Split-K partial sums: one device buffer, grown on demand and kept for the
+// life of the process.  Calls from several host threads at once are not
+// supported.
+float *g_part = nullptr;
+size_t g_part_bytes = 0;
+
+float *part_buffer(size_t bytes) {
+    if (bytes <= g_part_bytes) return g_part;
+    if (g_part) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_part);
+        g_part = nullptr;
+        g_part_bytes = 0;
+    }
+    if (hipMalloc((void **)&g_part, bytes) != hipSuccess) {
+        g_part = nullptr;
+        return nullptr;
+    }
+    g_part_bytes = bytes;
+    return g_part;
 }
 
 // W[co][t][ci] -> Wm[ci][26 - t][co] (mirror taps, swap channels): the
@@ -391,14 +435,30 @@ int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, flo
     const long long P = (long long)n * d * h * w;
     const int nbTotal = cout / 16;
     const int NB = nbTotal >= 4 ? 4 : nbTotal;
-    dim3 grid((unsigned)((P + 127) / 128), (unsigned)((nbTotal + NB - 1) / NB));
+    const long long base = ((P + 127) / 128) * ((nbTotal + NB - 1) / NB);
+    // split the taps over 3, 9 or 27 slices when the grid is far below the chip
+    int nsplit = 1;
+    while (nsplit < 27 && base * nsplit < 512) nsplit *= 3;
+    float *part = nullptr;
+    if (nsplit > 1) {
+        part = part_buffer(sizeof(float) * (size_t)nsplit * (size_t)P * (size_t)cout);
+        if (!part) {
+            fprintf(stderr, "warm-start conv forward: split-K buffer allocation failed\n");
+            return 0;
+        }
+    }
+    dim3 grid((unsigned)((P + 127) / 128), (unsigned)((nbTotal + NB - 1) / NB), (unsigned)nsplit);
     const bf16_t *x = (const bf16_t *)X, *wt = (const bf16_t *)W;
     bf16_t *y = (bf16_t *)Y;
     switch (NB) {
-    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
-    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
-    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
-    default: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, s, cin, cout, relu); break;
+    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
+    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
+    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
+    default: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
+    }
+    if (part) {
+        const long long m = P * cout;
+        hipLaunchKernelGGL(k_conv_epilogue, g1(m), dim3(256), 0, 0, part, nsplit, bias, y, Yf, m, cout, relu);
     }
     return ws_check("conv forward");
 }

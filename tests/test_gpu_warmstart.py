@@ -32,7 +32,9 @@ def dev_zeros(n, itemsize):
     return d
 
 
-SHAPES = [  # (n, d, h, w, cin, cout): tails of the 128-position tiles, 1..4 column blocks
+SHAPES = [  # (n, d, h, w, cin, cout): tails of the 128-position tiles, 1..4 column blocks;
+    # small grids split the taps over workgroup slices, the first shape does not
+    (4, 16, 32, 32, 32, 32),
     (2, 3, 5, 7, 32, 48),
     (1, 4, 4, 8, 64, 64),
     (1, 2, 2, 4, 256, 128),
