@@ -202,6 +202,16 @@ void LBM_CdMonitorFree(LBMCdMonitor *m);
  * Cd_friction, EMA of Cd_corr}; returns 1 when this sample first meets the
  * convergence test (relative std of the last 100 Cd samples < 1 %). */
 int LBM_CdMonitorSample(LBMCdMonitor *m, LBMGrid *grid, float out[6]);
+/* main.c:439-552 and 1273-1306: the model's world bounding box and Reynolds
+ * number; the next verbose sample prints the reference's one-time "Cd
+ * calculation breakdown" (Re, body size in cells, projected and bbox area,
+ * blockage and its correction, resolution and confinement warnings).  A
+ * projected area below 1 falls back to the bbox area, as main.c:548-549. */
+void LBM_CdMonitorSetBody(LBMCdMonitor *m, LBMGrid *grid, float reynolds, float minX, float minY, float minZ,
+                          float maxX, float maxY, float maxZ);
+/* main.c:524-537: Re = U L / nu with L the body length in cells and
+ * nu = (tau - 1/2)/3; print != 0 writes the "Effective Re = ..." line. */
+float LBM_EffectiveReynolds(LBMGrid *grid, float U, float minX, float maxX, int print);
 int LBM_CdMonitorConverged(const LBMCdMonitor *m, float *mean, float *relStd);
 /* Strouhal number of the monitor's Cl series (simulation/src/drag_metrics.c:8-68),
  * one sample per sampleInterval lattice steps. */

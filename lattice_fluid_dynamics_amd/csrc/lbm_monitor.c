@@ -32,6 +32,9 @@ struct LBMCdMonitor {
     float *cl;
     int clCount, clCap;
     int verbose;
+    /* body description for the one-time breakdown (LBM_CdMonitorSetBody) */
+    int hasBody, breakdownPrinted;
+    float reynolds, charLength, bodyY, bodyZ, bboxArea, epsilon;
 };
 
 LBMCdMonitor *LBM_CdMonitorCreate(LBMGrid *grid, float U, float refArea, int verbose) {
@@ -44,6 +47,50 @@ LBMCdMonitor *LBM_CdMonitorCreate(LBMGrid *grid, float U, float refArea, int ver
     m->blockageFactor = (1.0f - eps) * (1.0f - eps);
     m->verbose = verbose;
     return m;
+}
+
+/* main.c keeps the model's world bounding box (carBounds) and the Reynolds
+ * number; they feed the one-time "Cd calculation breakdown" of
+ * main.c:1273-1306, printed by the first verbose LBM_CdMonitorSample.
+ * Lattice extents use the world-to-grid scales of main.c:439 and 545-546
+ * (x: NX/8, y: NY/4, z: NZ/4). */
+void LBM_CdMonitorSetBody(LBMCdMonitor *m, LBMGrid *grid, float reynolds, float minX, float minY, float minZ,
+                          float maxX, float maxY, float maxZ) {
+    if (!m || !grid) return;
+    m->hasBody = 1;
+    m->reynolds = reynolds;
+    m->charLength = (maxX - minX) * (grid->sizeX / 8.0f);
+    m->bodyY = (maxY - minY) * (grid->sizeY / 4.0f);
+    m->bodyZ = (maxZ - minZ) * (grid->sizeZ / 4.0f);
+    m->bboxArea = m->bodyY * m->bodyZ;
+    /* main.c:547-552: projected area (bbox if none), blockage, correction */
+    if (m->refArea < 1.0f) m->refArea = m->bboxArea;
+    m->epsilon = m->refArea / ((float)grid->sizeY * grid->sizeZ);
+    m->blockageFactor = (1.0f - m->epsilon) * (1.0f - m->epsilon);
+}
+
+/* Reynolds number from the lattice inlet speed, the body length along x and
+ * nu = (tau - 1/2) / 3, with the "Effective Re" line of main.c:524-537. */
+float LBM_EffectiveReynolds(LBMGrid *grid, float U, float minX, float maxX, int print) {
+    if (!grid) return 0.0f;
+    const float charL = (maxX - minX) * (grid->sizeX / 8.0f);
+    const float nu = (grid->tau - 0.5f) / 3.0f;
+    const float re = (nu > 1e-10f) ? U * charL / nu : 0.0f;
+    if (print) printf("Effective Re = %.0f (U=%.3f, L=%.1f, nu=%.4f)\n", re, U, charL, nu);
+    return re;
+}
+
+static void print_breakdown(LBMCdMonitor *m) {
+    printf("  Cd calculation breakdown:\n");
+    printf("    Re = %.0f, U_lattice = %.4f, rho_0 = %.1f\n", m->reynolds, m->U, 1.0f);
+    printf("    Body (lattice): %.1f x %.1f x %.1f cells\n", m->charLength, m->bodyY, m->bodyZ);
+    printf("    Projected area = %.1f cells^2 (bbox = %.1f)\n", m->refArea, m->bboxArea);
+    printf("    Blockage = %.1f%%, correction = %.3f (Pope & Harper 1966)\n", m->epsilon * 100.0f,
+           m->blockageFactor);
+    if (fminf(m->bodyY, m->bodyZ) < 10.0f)
+        printf("    WARNING: body < 10 cells across -- Cd is unreliable at this resolution\n");
+    if (m->epsilon > 0.05f) printf("    WARNING: blockage > 5%% -- confinement inflates Cd\n");
+    printf("    NOTE: published Ahmed body Cd ~0.3 is at Re > 500k. Low-Re simulations give higher Cd.\n");
 }
 
 void LBM_CdMonitorFree(LBMCdMonitor *m) {
@@ -67,6 +114,10 @@ int LBM_CdMonitorSample(LBMCdMonitor *m, LBMGrid *grid, float out[6]) {
     float CdF = Cd - CdP;
     if (CdF < 0.0f) CdF = 0.0f;
     float CdCorr = Cd * m->blockageFactor;
+    if (m->verbose && m->hasBody && !m->breakdownPrinted) {
+        print_breakdown(m);
+        m->breakdownPrinted = 1;
+    }
     if (m->count == 0 && !m->converged && m->clCount == 0)
         m->ema = CdCorr;
     else

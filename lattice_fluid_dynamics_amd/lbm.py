@@ -90,6 +90,8 @@ SIGNATURES = {
     "LBM_CdMonitorCreate": (c_void_p, [GridP, c_float, c_float, c_int]),
     "LBM_CdMonitorFree": (None, [c_void_p]),
     "LBM_CdMonitorSample": (c_int, [c_void_p, GridP, c_float_p]),
+    "LBM_CdMonitorSetBody": (None, [c_void_p, GridP] + [c_float] * 7),
+    "LBM_EffectiveReynolds": (c_float, [GridP, c_float, c_float, c_float, c_int]),
     "LBM_CdMonitorConverged": (c_int, [c_void_p, c_float_p, c_float_p]),
     "LBM_CdMonitorStrouhal": (c_float, [c_void_p, c_int, c_float]),
     "LBM_ComputeStrouhal": (c_float, [c_float_p, c_int, c_int, c_float, c_float]),
@@ -386,6 +388,11 @@ class CdMonitor:
         keys = ("Cd", "Cd_corr", "Cl", "Cd_pressure", "Cd_friction", "ema")
         return bool(newly), dict(zip(keys, out.tolist()))
 
+    def set_body(self, reynolds: float, bbox):
+        """World bounding box (minX, minY, minZ, maxX, maxY, maxZ) and Re of
+        the model: the next verbose sample prints the Cd breakdown."""
+        load_library().LBM_CdMonitorSetBody(self.ptr, self.g.ptr, reynolds, *bbox)
+
     def converged(self):
         m, r = c_float(), c_float()
         ok = load_library().LBM_CdMonitorConverged(self.ptr, ctypes.byref(m), ctypes.byref(r))
@@ -406,6 +413,10 @@ class CdMonitor:
 def LBM_ComputeStrouhal(cl: np.ndarray, sample_interval: int, char_length: float, U: float) -> float:
     c = np.ascontiguousarray(cl, dtype=np.float32)
     return float(load_library().LBM_ComputeStrouhal(_fp(c), len(c), sample_interval, char_length, U))
+
+
+def LBM_EffectiveReynolds(g: Grid, U: float, minX: float, maxX: float, print_line: bool = True) -> float:
+    return float(load_library().LBM_EffectiveReynolds(g.ptr, U, minX, maxX, int(print_line)))
 
 
 def LBM_WriteVTI(g: Grid, directory: str, step: int) -> int:
