@@ -231,6 +231,10 @@ int LBM_WriteVTI(LBMGrid *grid, const char *dir, int step);
 int LBM_Autotune(LBMGrid *grid, int steps, float inletVelX, float inletVelY, float inletVelZ,
                  float *msPerStep);
 
+/* Sets the step kernel's workgroup size (128, 256 or 512) instead of
+ * tuning it; the results do not depend on it.  1 on success. */
+int LBM_SetBlockSize(LBMGrid *grid, int blockSize);
+
 /* Attainable HBM bandwidth in GB/s: read + write bytes of a plain device copy
  * of `bytes` bytes over its mean launch time (iters launches). */
 float LBM_MeasureCopyBandwidth(unsigned long long bytes, int iters);

@@ -918,6 +918,15 @@ int LBM_Autotune(LBMGrid *g, int steps, float ux, float uy, float uz, float *msP
     return best;
 }
 
+int LBM_SetBlockSize(LBMGrid *g, int bs) {
+    if (!g || !impl_of(g)->dev) return 0;
+    if (!lbmdev_set_block_size(impl_of(g)->dev, bs)) {
+        note_error("LBM_SetBlockSize: the workgroup size must be 128, 256 or 512");
+        return 0;
+    }
+    return 1;
+}
+
 /* Attainable HBM bandwidth: a plain device copy of `bytes` bytes, read +
  * write bytes over the mean launch time. */
 float LBM_MeasureCopyBandwidth(unsigned long long bytes, int iters) {

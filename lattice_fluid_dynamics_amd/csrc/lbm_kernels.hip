@@ -509,7 +509,11 @@ struct StepArgs {
 
 // One cell of the fused stream + collide update.  PHASE 0 = A, 1 = B (see
 // file header).
-template <int FMT, int OP, int PHASE>
+// Directions with ex = +1 and ex = -1, for the row-aligned phase-B stores.
+constexpr int XPLUS[5] = {1, 7, 9, 11, 13};
+constexpr int XMINUS[5] = {2, 8, 10, 12, 14};
+
+template <int FMT, int OP, int PHASE, int ROW = 0>
 __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell) {
     typedef Pop<FMT> P;
     typedef typename P::T T;
@@ -560,21 +564,61 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
             const int j = OPP[i];
             io.st(j, (uint32_t)cell, P::enc(fo[i], j));
         }
-    } else {
+    } else if (!ROW) {
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const uint32_t dst = (uint32_t)(X[1 + EX[i]] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
             io.st(i, dst, P::enc(fo[i], i));
         }
+    } else {
+        // Row-aligned phase-B stores (the workgroup holds whole x-rows, nx a
+        // multiple of 64).  Slot (i, x + e_i) is read in this phase only by
+        // cell x itself, so after a workgroup barrier any lane of the row may
+        // store it.  Lane l of a wave starting at row position x0 stores
+        // slot (i, x0 + l): for ex_i = +1 the value of cell x0 + l - 1 (the
+        // lane below, or lane 63 of the previous wave of the row, wrapping),
+        // for ex_i = -1 that of cell x0 + l + 1; every store then covers
+        // whole cache lines.
+        __shared__ T xp[8][5], xm[8][5];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int wpr = g.nx >> 6, rw = w % wpr, w0 = w - rw;
+        const int wprev = rw == 0 ? w0 + wpr - 1 : w - 1, wnext = rw == wpr - 1 ? w0 : w + 1;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            if (lane == 63) xp[w][k] = fo[XPLUS[k]];
+            if (lane == 0) xm[w][k] = fo[XMINUS[k]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            const uint32_t dst = (uint32_t)(X[1] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
+            T v = fo[i];
+            if (EX[i] == 1) {
+                v = __shfl_up(v, 1, 64);
+                int k = 0;
+#pragma unroll
+                for (int j = 0; j < 5; j++) k = XPLUS[j] == i ? j : k;
+                if (lane == 0) v = xp[wprev][k];
+            } else if (EX[i] == -1) {
+                v = __shfl_down(v, 1, 64);
+                int k = 0;
+#pragma unroll
+                for (int j = 0; j < 5; j++) k = XMINUS[j] == i ? j : k;
+                if (lane == 63) v = xm[wnext][k];
+            }
+            io.st(i, dst, P::enc(v, i));
+        }
     }
 }
 
 // The fused kernel over one lattice, BS lanes per workgroup.
-template <int FMT, int OP, int PHASE, int BS>
+// ROW = 1 (phase B only): row-aligned stores; the launch guarantees that
+// every workgroup holds whole rows and no lane is past the lattice.
+template <int FMT, int OP, int PHASE, int BS, int ROW = 0>
 __global__ __launch_bounds__(BS) void k_step(StepArgs<FMT> a) {
     const int cell = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if (cell >= a.g.ncell) return;
-    step_cell<FMT, OP, PHASE>(a, cell);
+    if (!ROW && cell >= a.g.ncell) return;
+    step_cell<FMT, OP, PHASE, ROW>(a, cell);
 }
 
 // ---------------------------------------------------------- batched tunnels
@@ -606,12 +650,13 @@ struct BatchArgs {
     const int32_t *block_tunnel;
 };
 
-template <int FMT, int OP, int PHASE, int BS>
+// ROW = 1 (phase B only): every tunnel's rows fill whole workgroups.
+template <int FMT, int OP, int PHASE, int BS, int ROW = 0>
 __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const TunnelDesc d = b.desc[b.block_tunnel[blk]];
     const int cell = (blk - d.blockStart) * BS + threadIdx.x;
-    if (cell >= d.g.ncell) return;
+    if (!ROW && cell >= d.g.ncell) return;
     StepArgs<FMT> a;
     a.pop = b.pop + d.popOff;
     a.mask = b.mask + d.cellOff;
@@ -628,7 +673,7 @@ __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     a.cp.iny = d.iny;
     a.cp.inz = d.inz;
     a.cp.smag = b.smag;
-    step_cell<FMT, OP, PHASE>(a, cell);
+    step_cell<FMT, OP, PHASE, ROW>(a, cell);
 }
 
 template <int FMT>
@@ -1314,8 +1359,12 @@ static void launch_prepass(LbmDev *d, int periodic) {
 template <int FMT, int OP, int BS>
 static void launch_step_phase(int phase, int ncell, hipStream_t st, const StepArgs<FMT> &a) {
     const int nb = (ncell + BS - 1) / BS;
+    // whole rows per workgroup, rows of whole wavefronts, no partial workgroup
+    const bool row = FMT != LBMDEV_FP64 && a.g.nx % 64 == 0 && BS % a.g.nx == 0 && ncell % BS == 0;
     if (phase == 0)
         hipLaunchKernelGGL((k_step<FMT, OP, 0, BS>), dim3(nb), dim3(BS), 0, st, a);
+    else if (row)
+        hipLaunchKernelGGL((k_step<FMT, OP, 1, BS, 1>), dim3(nb), dim3(BS), 0, st, a);
     else
         hipLaunchKernelGGL((k_step<FMT, OP, 1, BS>), dim3(nb), dim3(BS), 0, st, a);
 }
@@ -1998,8 +2047,16 @@ static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase
     a.smag = p->useSmagorinsky == 1;
     a.desc = b->desc_d;
     a.block_tunnel = b->block_tunnel;
+    // row-aligned phase B when every tunnel's rows fill whole workgroups
+    bool row = FMT != LBMDEV_FP64;
+    for (int t = 0; t < b->nt && row; t++) {
+        const Geo &g = b->desc_h[t].g;
+        row = g.nx % 64 == 0 && BS % g.nx == 0 && g.ncell % BS == 0;
+    }
     if (phase == 0)
         hipLaunchKernelGGL((k_step_batch<FMT, OP, 0, BS>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+    else if (row)
+        hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS, 1>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
     else
         hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
 }

@@ -97,6 +97,7 @@ SIGNATURES = {
     "LBM_ComputeStrouhal": (c_float, [c_float_p, c_int, c_int, c_float, c_float]),
     "LBM_WriteVTI": (c_int, [GridP, ctypes.c_char_p, c_int]),
     "LBM_Autotune": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p]),
+    "LBM_SetBlockSize": (c_int, [GridP, c_int]),
     "LBM_MeasureCopyBandwidth": (c_float, [ctypes.c_ulonglong, c_int]),
     "LBM_BatchCreate": (c_void_p, [c_int, c_int, c_int, c_int, c_float, c_int]),
     "LBM_BatchFree": (None, [c_void_p]),
@@ -421,6 +422,11 @@ def LBM_EffectiveReynolds(g: Grid, U: float, minX: float, maxX: float, print_lin
 
 def LBM_WriteVTI(g: Grid, directory: str, step: int) -> int:
     return int(load_library().LBM_WriteVTI(g.ptr, directory.encode(), step))
+
+
+def LBM_SetBlockSize(g: Grid, bs: int) -> None:
+    if not load_library().LBM_SetBlockSize(g.ptr, bs):
+        raise ValueError(last_error())
 
 
 def LBM_Autotune(g: Grid, steps: int, ux: float, uy: float = 0.0, uz: float = 0.0):

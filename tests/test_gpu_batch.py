@@ -103,3 +103,27 @@ def test_batch_coefficients(gpu):
         assert c[k, 0] == pytest.approx(abs(F[k, 0]) / den, rel=1e-6)
         assert c[k, 1] == pytest.approx(F[k, 1] / den, rel=1e-5, abs=1e-7)
         assert c[k, 0] > 0
+
+
+def test_batch_row_aligned_equals_per_cell_stores(gpu):
+    """Tunnels whose rows fill whole workgroups take the row-aligned phase-B
+    stores at workgroup 256 and per-cell stores at 128 (128 does not divide
+    the 256-cell rows); both give the same bits, FP32 and FP16 storage."""
+    tunnels = [((64, 32, 32), (-1.0, 0.1, 0.0, 0.7), 0.56, 0.05),
+               ((128, 16, 16), (-0.5, -0.2, 0.1, 0.5), 0.62, 0.03),
+               ((256, 16, 8), (0.3, 0.0, 0.2, 0.9), 0.80, 0.07)]
+    for storage in (0, 1):
+        res = []
+        for bs in (256, 128):
+            b = gpu.Batch(storage=storage, useRegularized=1, useMRT=0, useSmagorinsky=1, smagorinskyCs=0.1,
+                          periodicYZ=1)
+            for dims, sph, tau, u in tunnels:
+                geo = make_geometry(gpu, dims, sph)
+                b.add(geo, tau, u)
+                geo.free()
+            b.initialize()
+            b.set_block_size(bs)
+            b.step(9)
+            res.append([b.populations(k, int(np.prod(d))) for k, (d, _, _, _) in enumerate(tunnels)])
+        for (fa, fna), (fb, fnb) in zip(res[0], res[1]):
+            assert np.array_equal(fa, fb) and np.array_equal(fna, fnb), storage

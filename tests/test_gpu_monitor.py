@@ -47,23 +47,35 @@ def test_monitor_algebra_and_ema(gpu):
 
 
 def test_monitor_converges_on_a_settled_flow(gpu):
-    g = sphere_grid()
+    """The convergence flag fires the first time the relative standard
+    deviation of the last 100 Cd samples drops below 1 %, and reports that
+    window's mean and relStd (population standard deviation)."""
+    g = lbm.LBM_Create(64, 32, 32, 0.1)
+    g.tau = 0.8
+    g.periodicYZ = 1
+    lbm.LBM_SetSolidSphere(g, -2.0, 0.1, 0.0, 0.6)
+    assert lbm.LBM_InitializeFlow(g, U, 0, 0) == 1
     A = lbm.LBM_ComputeProjectedArea(g, 0)
-    lbm.LBM_StepN(g, 8000, U, 0, 0)
+    lbm.LBM_StepN(g, 30000, U, 0, 0)
     mon = lbm.CdMonitor(g, U, A)
     cds = []
     newly_at = None
-    for k in range(110):
+    for k in range(300):
         lbm.LBM_StepN(g, 10, U, 0, 0)
         newly, s = mon.sample()
         cds.append(s["Cd"])
         if newly:
+            assert newly_at is None  # reported once
             newly_at = k
     ok, mean, rel = mon.converged()
-    assert ok and newly_at == 99  # first moment the 100-sample window is full
-    window = np.asarray(cds[:100], np.float64)
+    assert ok and newly_at is not None and newly_at >= 99
+    window = np.asarray(cds[newly_at - 99:newly_at + 1], np.float64)
     assert mean == pytest.approx(window.mean(), rel=1e-5)
     assert rel == pytest.approx(window.std() / window.mean(), rel=1e-3, abs=1e-6)
+    assert rel < 0.01
+    if newly_at > 99:  # the window before did not qualify
+        prev = np.asarray(cds[newly_at - 100:newly_at], np.float64)
+        assert prev.std() / prev.mean() >= 0.01 * (1 - 1e-3)
 
 
 def test_breakdown_and_effective_re_lines(gpu, capfd):

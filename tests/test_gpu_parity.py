@@ -314,3 +314,27 @@ def test_fp64_algorithm_exact_all_boundaries(gpu, case, periodic, op, smag):
     ex_o, _ = og.force()
     F = gpu.LBM_ComputeDragForceF64(g)
     assert np.allclose(F[:3], ex_o[:3], rtol=1e-8 if smag else 1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("op", ["reg", "mrt"])
+def test_row_aligned_phase_b_is_bitwise_identical(gpu, op):
+    """Workgroup 512 on a 512-cell row takes the row-aligned phase-B stores
+    (values moved between lanes before storing); workgroups 256 and 128 take
+    the per-cell stores.  Fields must agree bit for bit, boundary list
+    included (sphere with Bouzidi links, clamped y/z faces)."""
+    out = []
+    for bs in (512, 256, 128):
+        g = gpu.LBM_Create(512, 16, 16, 0.02)
+        g.tau = 0.56
+        g.useRegularized = int(op == "reg")
+        g.useMRT = int(op == "mrt")
+        g.useSmagorinsky = 1
+        g.smagorinskyCs = 0.1
+        g.periodicYZ = 0
+        gpu.LBM_SetSolidSphere(g, -1.0, 0.0, 0.0, 0.7)
+        assert gpu.LBM_InitializeFlow(g, 0.05, 0, 0) == 1
+        gpu.LBM_SetBlockSize(g, bs)
+        gpu.LBM_StepN(g, 7, 0.05, 0, 0)
+        out.append(fields(g))
+    for k in (1, 2):
+        assert all(np.array_equal(a, b) for a, b in zip(out[0], out[k])), k
