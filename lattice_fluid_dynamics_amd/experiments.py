@@ -119,7 +119,7 @@ def _steps_to_1pct(h, ref, window):
 def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 6000, avg_steps: int = 2000,
                          cold_steps: int = 20000, warm_steps: int = 12000, batch: int = 16, lr: float = 1e-3,
                          weights_in=None, weights_out=None, chunk: int = 96, window: int = 20,
-                         dims=(256, 128, 128), factor: int = 4, phase: str = "all"):
+                         dims=(256, 128, 128), factor: int = 4, phase: str = "all", warm_fp32: bool = True):
     """Config 5: learned warm start at 256x128x128 (coarse lattice 64x32x32).
 
     Training tunnels use body seeds 0..n_train-1, held-out tunnels seeds
@@ -167,7 +167,8 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
     runs["cold_fp32"] = WS.cd_histories(specs, dims, cold_steps, 0, None, factor, avg_from=cold_steps - 6000, log=log)
     runs["cold_fp16"] = WS.cd_histories(specs, dims, warm_steps, 1, None, factor, log=log)
     runs["warm_fp16"] = WS.cd_histories(specs, dims, warm_steps, 1, ws, factor, log=log)
-    runs["warm_fp32"] = WS.cd_histories(specs, dims, warm_steps, 0, ws, factor, log=log)
+    if warm_fp32:
+        runs["warm_fp32"] = WS.cd_histories(specs, dims, warm_steps, 0, ws, factor, log=log)
     conv = runs["cold_fp32"]["mean_field"]
     rows = []
     for t, sp in enumerate(specs):
@@ -224,6 +225,7 @@ def main(argv=None):
     ap.add_argument("--weights-in", default=None)
     ap.add_argument("--weights-out", default=None)
     ap.add_argument("--phase", choices=["all", "train", "eval"], default="all")
+    ap.add_argument("--no-warm-fp32", action="store_true", help="warmstart: skip the warm-start FP32 runs")
     a = ap.parse_args(argv)
     if a.which == "storage":
         storage_experiment(a.steps or 20000)
@@ -232,7 +234,7 @@ def main(argv=None):
     else:
         warmstart_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps, cold_steps=a.cold_steps,
                              warm_steps=a.warm_steps, weights_in=a.weights_in, weights_out=a.weights_out,
-                             phase=a.phase)
+                             phase=a.phase, warm_fp32=not a.no_warm_fp32)
 
 
 if __name__ == "__main__":
