@@ -21,6 +21,7 @@
 
 #include "../../include/lbm.h"
 #include "lbm_device.h"
+#include "lbm_mesh.h"
 
 /* D3Q19 velocity set, reference order (simulation/src/lbm.c:9-35). */
 static const int kEx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
@@ -187,11 +188,7 @@ void LBM_Free(LBMGrid *grid) {
 /* ------------------------------------------------------------ geometry */
 
 /* World box x in [-4,4], y,z in [-2,2] mapped onto the lattice
- * (simulation/src/lbm.c:352-354). */
-typedef struct {
-    float sx, sy, sz;
-} Scale;
-
+ * (simulation/src/lbm.c:352-354); Scale and cell_centre are in lbm_mesh.h. */
 static Scale world_scale(const LBMGrid *g) {
     Scale s = {g->sizeX / 8.0f, g->sizeY / 4.0f, g->sizeZ / 4.0f};
     return s;
@@ -257,18 +254,6 @@ typedef struct {
     float cx, cy, cz, r;
 } SphereCtx;
 
-/* Cell centre in world coordinates (simulation/src/lbm.c:446-448). */
-static void cell_centre(Scale s, int x, int y, int z, float *wx, float *wy, float *wz) {
-    *wx = (x + 0.5f) / s.sx - 4.0f;
-    *wy = (y + 0.5f) / s.sy - 2.0f;
-    *wz = (z + 0.5f) / s.sz - 2.0f;
-}
-
-static float clamp_q(float q) {
-    if (q < 0.01f) q = 0.01f;
-    if (q > 0.99f) q = 0.99f;
-    return q;
-}
 
 /* Nearest root of |o + t d|^2 = r^2 with t in (0,1], else 0.5
  * (simulation/src/lbm.c:492-515). */
@@ -375,29 +360,10 @@ float LBM_ComputeProjectedArea(LBMGrid *g, int axis) {
 }
 
 /* ---- triangle mesh ---------------------------------------------------------
- * Solid: odd number of +X ray crossings from the cell centre, over triangles
- * whose (y, z) range contains the centre and whose max x is not behind it
- * (simulation/src/lbm.c:1006-1097).  q: nearest intersection t in (0,1] along
- * each fluid->solid link, over the SAME candidate set (the reference reuses
- * the +X cull for every link direction, simulation/src/lbm.c:1150-1157),
- * fallback 0.5, clamped to [0.01, 0.99].
- * A uniform (y,z) bin grid narrows the candidates; the exact cull test is
- * still applied per triangle, so the candidate set (and every result) is the
- * same as testing all triangles. */
-typedef struct {
-    const float *tri;
-    int ntri;
-    float *bounds; /* per triangle: minY maxY minZ maxZ maxX */
-    int by, bz;
-    float y0, z0, iy, iz;
-    int *binStart, *binList;
-} MeshIndex;
-
-static int bin_of(float v, float o, float inv, int n) {
-    int b = (int)floorf((v - o) * inv);
-    return b < 0 ? 0 : (b >= n ? n - 1 : b);
-}
-
+ * Semantics in lbm_mesh.h (simulation/src/lbm.c:964-1223).  The host builds
+ * the (y, z) bin index of the triangles; the GPU marks the solid cells,
+ * finds the cells with links and evaluates their q (mesh_kernels.hip), with
+ * the float expressions the host would use. */
 static int mesh_index_build(MeshIndex *m, const float *tri, int ntri) {
     memset(m, 0, sizeof(*m));
     m->tri = tri;
@@ -464,72 +430,6 @@ static void mesh_index_free(MeshIndex *m) {
     free(m->binList);
 }
 
-/* Ray-triangle test (Moller & Trumbore 1997) in the float expression order
- * the reference uses; returns the ray parameter through *t.  The +X solid
- * test specialises h = d x e2 and v = f (d . q) for d = (1,0,0). */
-static int ray_tri(const float *v, float ox, float oy, float oz, float dx, float dy, float dz, int plusX,
-                   float eps, float *t) {
-    float e1x = v[4] - v[0], e1y = v[5] - v[1], e1z = v[6] - v[2];
-    float e2x = v[8] - v[0], e2y = v[9] - v[1], e2z = v[10] - v[2];
-    float hx, hy, hz;
-    if (plusX) {
-        hx = 0;
-        hy = -e2z;
-        hz = e2y;
-    } else {
-        hx = dy * e2z - dz * e2y;
-        hy = dz * e2x - dx * e2z;
-        hz = dx * e2y - dy * e2x;
-    }
-    float a = e1x * hx + e1y * hy + e1z * hz;
-    if (a > -eps && a < eps) return 0;
-    float f = 1.0f / a;
-    float sx = ox - v[0], sy = oy - v[1], sz = oz - v[2];
-    float u = f * (sx * hx + sy * hy + sz * hz);
-    if (u < 0.0f || u > 1.0f) return 0;
-    float qx = sy * e1z - sz * e1y;
-    float qy = sz * e1x - sx * e1z;
-    float qz = sx * e1y - sy * e1x;
-    float vv = plusX ? f * qx : f * (dx * qx + dy * qy + dz * qz);
-    if (vv < 0.0f || u + vv > 1.0f) return 0;
-    *t = f * (e2x * qx + e2y * qy + e2z * qz);
-    return 1;
-}
-
-/* Candidate triangles for a cell centre: the bin's list filtered by the
- * reference's cull test. */
-#define FOR_CANDIDATES(m, wx, wy, wz, T)                                                        \
-    for (int _k = (m)->binStart[bin_of(wy, (m)->y0, (m)->iy, (m)->by) +                       \
-                                (m)->by * bin_of(wz, (m)->z0, (m)->iz, (m)->bz)],              \
-             _e = (m)->binStart[bin_of(wy, (m)->y0, (m)->iy, (m)->by) +                        \
-                                (m)->by * bin_of(wz, (m)->z0, (m)->iz, (m)->bz) + 1];          \
-         _k < _e; _k++)                                                                         \
-        if ((T = (m)->binList[_k]), !((wy) < (m)->bounds[5L * T + 0] || (wy) > (m)->bounds[5L * T + 1] || \
-                                       (wz) < (m)->bounds[5L * T + 2] || (wz) > (m)->bounds[5L * T + 3] || \
-                                       (wx) > (m)->bounds[5L * T + 4]))
-
-typedef struct {
-    Scale s;
-    MeshIndex *m;
-} MeshCtx;
-
-static float mesh_q(void *vctx, int x, int y, int z, int i) {
-    MeshCtx *c = (MeshCtx *)vctx;
-    float wx, wy, wz;
-    cell_centre(c->s, x, y, z, &wx, &wy, &wz);
-    float dx = (float)kEx[i] / c->s.sx, dy = (float)kEy[i] / c->s.sy, dz = (float)kEz[i] / c->s.sz;
-    float best = 2.0f;
-    int t;
-    if (c->m->ntri > 0) {
-        FOR_CANDIDATES(c->m, wx, wy, wz, t) {
-            float th;
-            if (ray_tri(c->m->tri + 12L * t, wx, wy, wz, dx, dy, dz, 0, 1e-6f, &th) && th > 0.0f && th < best)
-                best = th;
-        }
-    }
-    return clamp_q(best <= 1.0f ? best : 0.5f);
-}
-
 /* simulation/src/lbm.c:964-1223 */
 void LBM_SetSolidMesh(LBMGrid *g, float *triangles, int numTriangles, float minX, float minY, float minZ,
                       float maxX, float maxY, float maxZ) {
@@ -541,34 +441,30 @@ void LBM_SetSolidMesh(LBMGrid *g, float *triangles, int numTriangles, float minX
         note_error("LBM_SetSolidMesh: host allocation failed");
         return;
     }
-    Scale s = world_scale(g);
-    memset(im->solid, 0, sizeof(int32_t) * (size_t)g->totalCells);
-    int count = 0;
-
-    for (int z = 0; z < g->sizeZ; z++) {
-        for (int y = 0; y < g->sizeY; y++)
-            for (int x = 0; x < g->sizeX; x++) {
-                float wx, wy, wz;
-                cell_centre(s, x, y, z, &wx, &wy, &wz);
-                if (wx < minX || wx > maxX || wy < minY || wy > maxY || wz < minZ || wz > maxZ) continue;
-                if (m.ntri == 0) continue;
-                int hits = 0, t;
-                FOR_CANDIDATES(&m, wx, wy, wz, t) {
-                    float d;
-                    if (ray_tri(triangles + 12L * t, wx, wy, wz, 1.0f, 0.0f, 0.0f, 1, 0.00001f, &d) &&
-                        d > 0.00001f)
-                        hits++;
-                }
-                if (hits % 2 == 1) {
-                    im->solid[cell_index(g, x, y, z)] = 1;
-                    count++;
-                }
-            }
-    }
-    printf("LBM mesh solid: %d cells marked as solid\n", count);
+    const float box[6] = {minX, minY, minZ, maxX, maxY, maxZ};
+    int n = 0, *cells = NULL;
+    float *qr = NULL;
     q_reset(im, g->totalCells);
-    MeshCtx ctx = {s, &m};
-    mark_links(g, not_body, not_fluid, mesh_q, &ctx);
+    if (!lbmdev_mesh_voxelize(g->sizeX, g->sizeY, g->sizeZ, &m, world_scale(g), box, im->solid, &n, &cells, &qr)) {
+        mesh_index_free(&m);
+        memset(im->solid, 0, sizeof(int32_t) * (size_t)g->totalCells);
+        note_error("LBM_SetSolidMesh: GPU voxelisation failed");
+        geometry_changed(g);
+        return;
+    }
+    long count = 0;
+    for (long c = 0; c < g->totalCells; c++) count += im->solid[c] == 1;
+    printf("LBM mesh solid: %ld cells marked as solid\n", count);
+    for (int k = 0; k < n; k++) { /* rows in cell order, as the host link walk creates them */
+        float *row = q_row_for(im, cells[k], 1);
+        if (!row) {
+            note_error("LBM_SetSolidMesh: host allocation failed");
+            break;
+        }
+        memcpy(row, qr + 19L * k, sizeof(float) * 19);
+    }
+    free(cells);
+    free(qr);
     printf("Bouzidi: %d boundary cells with links\n", im->nRows);
     mesh_index_free(&m);
     geometry_changed(g);
