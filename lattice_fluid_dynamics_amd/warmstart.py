@@ -462,3 +462,11 @@ def cd_histories(specs: Sequence[TunnelSpec], dims, steps: int, storage: int, ws
     del b
     return {"histories": hs, "ms_per_step": ms / steps, "init_s": init_s, "predicted": predicted,
             "mean_field": None if acc is None else (acc / nav).astype(np.float32), "masks": masks}
+
+
+if __name__ == "__main__":
+    # python -m lattice_fluid_dynamics_amd.warmstart [iters]: training-step benchmark
+    import json
+    import sys
+
+    print(json.dumps(train_benchmark(iters=int(sys.argv[1]) if len(sys.argv) > 1 else 50)), flush=True)
