@@ -267,6 +267,247 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t *__restrict__ d
     if (do_bias && tid < TCO) atomicAdd(&db[co0 + tid], bsum);
 }
 
+// Brick-tiled weight gradient for 32 x 32 (co x ci) blocks, used when the
+// spatial extent divides into 8 x 4 x 4 bricks.  A brick of 128 positions is
+// four MFMA K-slabs (one per z plane of the brick) for all 27 taps, so each
+// input element is fetched once per brick plus a 2.8x halo instead of once
+// per tap.  Per brick:
+//   * lanes 0..143 fetch the brick's 10 x 6 x 6 X halo (one (hz, hy) row of
+//     10 positions x 8 channels each), transpose it in registers and store
+//     three copies shifted by dx = -1, 0, +1: Xs[dx+1][ci][hz][hy][bx];
+//   * lanes 192..255 fetch the 8 x 4 x 4 dZ tile (one (by, bz) row x 8
+//     channels each) and store it transposed, Ds[co][k] with
+//     k = bx + 8 (by + 4 bz); they also accumulate the bias gradient;
+//   * wave w owns the 16 x 16 block (w / 2, w % 2) of all 27 taps.  In slab
+//     bz, MFMA lane group kq holds row by = kq: one A read per slab and, per
+//     tap, one aligned 16-byte B read (8 consecutive bx of halo row
+//     (bz + dz + 1, kq + dy + 1)) into a 16x16x32 MFMA.
+// The next brick's fetch is issued before the current brick's MFMAs.  LDS
+// rows are padded so the 16 rows read by a wave start on distinct bank
+// groups.  Each workgroup writes its partial sums to
+// part[chunk][zblock][27][wave][j][lane] and, for the first ci block, its
+// bias partials to bpart[chunk][co]; k_wgrad_reduce adds them in a fixed
+// order (no atomics: bias atomics from every workgroup to the same 32
+// addresses serialise).
+constexpr int BR_ROWS = 36;                  // halo rows (hz, hy), 6 x 6
+constexpr int BR_XS_CI = BR_ROWS * 8 + 8;    // bf16 per ci row of one shifted copy (+ pad)
+constexpr int BR_DS_LD = 128 + 8;            // bf16 per co row of the dZ tile (+ pad)
+constexpr int BR_HALO_LANES = BR_ROWS * 4;   // 144
+constexpr int BR_DZ_LANE0 = 192;             // lanes 192..255 fetch dZ
+
+__device__ __forceinline__ uint32_t word_of(const uint4 &u, int k) {
+    return k == 0 ? u.x : (k == 1 ? u.y : (k == 2 ? u.z : u.w));
+}
+
+// channel j of positions a (low half) and b (high half)
+__device__ __forceinline__ uint32_t chan_pair(const uint4 &a, const uint4 &b, int j) {
+    const uint32_t wa = word_of(a, j >> 1), wb = word_of(b, j >> 1);
+    return (j & 1) ? ((wa >> 16) | (wb & 0xffff0000u)) : ((wa & 0xffffu) | (wb << 16));
+}
+
+struct BrickCtx {
+    int bw, bh, bps;
+};
+
+__device__ __forceinline__ void brick_fetch(const bf16_t *__restrict__ dZ, const bf16_t *__restrict__ X,
+                                            const Shape &s, const BrickCtx &c, int b, int tid, int cin, int cout,
+                                            int co0, int ci0, uint4 (&v)[10]) {
+    const int sn = b / c.bps, loc = b % c.bps;
+    const int x0 = (loc % c.bw) * 8, y0 = ((loc / c.bw) % c.bh) * 4, z0 = (loc / (c.bw * c.bh)) * 4;
+    const long long sbase = (long long)sn * s.d * s.h * s.w;
+    if (tid < BR_HALO_LANES) {
+        const int row = tid >> 2, cg = tid & 3;
+        const int Y_ = y0 + row % 6 - 1, Z_ = z0 + row / 6 - 1;
+        const bool in = Y_ >= 0 && Y_ < s.h && Z_ >= 0 && Z_ < s.d;
+        const long long rowbase = sbase + ((long long)Z_ * s.h + Y_) * s.w;
+#pragma unroll
+        for (int hx = 0; hx < 10; hx++) {
+            const int X_ = x0 + hx - 1;
+            v[hx] = make_uint4(0, 0, 0, 0);
+            if (in && X_ >= 0 && X_ < s.w) v[hx] = *(const uint4 *)(X + (rowbase + X_) * cin + ci0 + 8 * cg);
+        }
+    } else if (tid >= BR_DZ_LANE0) {
+        const int row = (tid - BR_DZ_LANE0) >> 2, cg = tid & 3;  // row = by + 4 bz
+        const bf16_t *src =
+            dZ + (sbase + ((long long)(z0 + (row >> 2)) * s.h + y0 + (row & 3)) * s.w + x0) * cout + co0 + 8 * cg;
+#pragma unroll
+        for (int bx = 0; bx < 8; bx++) v[bx] = *(const uint4 *)(src + (long long)bx * cout);
+    }
+}
+
+__device__ __forceinline__ void brick_store(bf16_t *Ds, bf16_t *Xs, int tid, const uint4 (&v)[10], bool do_bias,
+                                            float (&bs)[8]) {
+    if (tid < BR_HALO_LANES) {
+        const int row = tid >> 2, cg = tid & 3;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            uint32_t w[9];
+#pragma unroll
+            for (int p = 0; p < 9; p++) w[p] = chan_pair(v[p], v[p + 1], j);
+#pragma unroll
+            for (int cx = 0; cx < 3; cx++)
+                *(uint4 *)(Xs + (cx * 32 + 8 * cg + j) * BR_XS_CI + row * 8) =
+                    make_uint4(w[cx], w[cx + 2], w[cx + 4], w[cx + 6]);
+        }
+    } else if (tid >= BR_DZ_LANE0) {
+        const int row = (tid - BR_DZ_LANE0) >> 2, cg = tid & 3;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            uint32_t w[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) w[m] = chan_pair(v[2 * m], v[2 * m + 1], j);
+            *(uint4 *)(Ds + (8 * cg + j) * BR_DS_LD + 8 * row) = make_uint4(w[0], w[1], w[2], w[3]);
+            if (do_bias) {
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    bs[j] += __uint_as_float(w[m] << 16) + __uint_as_float(w[m] & 0xffff0000u);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_conv_wgrad_brick(const bf16_t *__restrict__ dZ,
+                                                          const bf16_t *__restrict__ X, float *__restrict__ part,
+                                                          float *__restrict__ bpart, Shape s, int cin, int cout,
+                                                          int bricks_per_wg) {
+    __shared__ __align__(16) bf16_t Ds[32 * BR_DS_LD];
+    __shared__ __align__(16) bf16_t Xs[3 * 32 * BR_XS_CI];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    const int nbci = cin / 32;
+    const int zb = (cout / 32) * nbci;
+    const int co0 = (blockIdx.z / nbci) * 32, ci0 = (blockIdx.z % nbci) * 32;
+    BrickCtx c;
+    c.bw = s.w / 8;
+    c.bh = s.h / 4;
+    c.bps = c.bw * c.bh * (s.d / 4);
+    const int nbricks = s.n * c.bps;
+    const int b0 = blockIdx.x * bricks_per_wg, b1 = min(nbricks, b0 + bricks_per_wg);
+    const bool do_bias = bpart != nullptr && ci0 == 0;
+    const int bco = 16 * (wave >> 1), bci = 16 * (wave & 1);
+    f32x4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++) acc[t] = f32x4{0, 0, 0, 0};
+    float bs[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) bs[j] = 0.0f;
+    uint4 v[10];
+    if (b0 < b1) brick_fetch(dZ, X, s, c, b0, tid, cin, cout, co0, ci0, v);
+    for (int b = b0; b < b1; b++) {
+        brick_store(Ds, Xs, tid, v, do_bias, bs);
+        __syncthreads();
+        if (b + 1 < b1) brick_fetch(dZ, X, s, c, b + 1, tid, cin, cout, co0, ci0, v);
+#pragma unroll 1
+        for (int bz = 0; bz < 4; bz++) {
+            Pack8 a;
+            a.u = *(const uint4 *)(Ds + (bco + r) * BR_DS_LD + 32 * bz + 8 * kq);
+            // per dz plane: the 9 B operands are read back to back, then
+            // their 9 MFMAs run (one LDS latency per plane, not per tap)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) {
+                Pack8 bb[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    const int row = (bz + dz) * 6 + (kq + k / 3);
+                    bb[k].u = *(const uint4 *)(Xs + ((k % 3) * 32 + bci + r) * BR_XS_CI + row * 8);
+                }
+#pragma unroll
+                for (int k = 0; k < 9; k++)
+                    acc[9 * dz + k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bb[k].v, acc[9 * dz + k], 0, 0, 0);
+            }
+            // keep that order through scheduling: A read, then per plane
+            // 9 LDS reads followed by 9 MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 9, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 9, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 9, 0);
+        }
+        __syncthreads();
+    }
+    float *pw = part + ((long long)blockIdx.x * zb + blockIdx.z) * 27 * 1024;
+#pragma unroll
+    for (int t = 0; t < 27; t++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) pw[((t * 4 + wave) * 4 + j) * 64 + lane] = acc[t][j];
+    if (do_bias) {
+        // the 16 (by, bz) rows of each channel group meet in LDS, summed in row order
+        float *red = (float *)Xs;
+        if (tid >= BR_DZ_LANE0) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) red[(tid - BR_DZ_LANE0) * 8 + j] = bs[j];
+        }
+        __syncthreads();
+        if (tid < 32) {
+            const int cg = tid >> 3, j = tid & 7;
+            float v = 0.0f;
+#pragma unroll
+            for (int row = 0; row < 16; row++) v += red[(row * 4 + cg) * 8 + j];
+            bpart[(long long)blockIdx.x * cout + co0 + tid] = v;
+        }
+    }
+}
+
+// Sums the brick partials over chunks: thread i handles part element i of
+// every (chunk, zblock) in stored order (contiguous loads), i.e. tap t,
+// wave w, register j, lane l of 32 x 32 block zb, which is
+// dW[co0 + 16 (w / 2) + 4 (l / 16) + j][t][ci0 + 16 (w % 2) + l % 16].
+// Chunks are split over RED_SLICES lane groups of a workgroup (8 loads in
+// flight each) and the slice sums are added in slice order: a fixed order
+// for any chunk count, so results do not depend on timing.  The last
+// workgroups do the same for the bias partials.
+constexpr int RED_SLICES = 4;
+
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ part, const float *__restrict__ bpart,
+                                                      int nchunks, int cin, int cout, float *__restrict__ dW,
+                                                      float *__restrict__ db) {
+    __shared__ float acc[RED_SLICES][64];
+    const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const long long i = (long long)blockIdx.x * 64 + o;
+    const int zb = (cout / 32) * (cin / 32);
+    const long long n = (long long)zb * 27 * 1024;
+    const bool bias = i >= n && i < n + cout && bpart != nullptr;
+    const float *p = nullptr;
+    long long stride = 0;
+    if (i < n) {
+        p = part + i;
+        stride = n;
+    } else if (bias) {
+        p = bpart + (i - n);
+        stride = cout;
+    }
+    float v = 0.0f;
+    if (p) {
+        int c = q;
+        for (; c + 7 * RED_SLICES < nchunks; c += 8 * RED_SLICES) {
+            float x[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = p[(long long)(c + k * RED_SLICES) * stride];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v += x[k];
+        }
+        for (; c < nchunks; c += RED_SLICES) v += p[(long long)c * stride];
+    }
+    acc[q][o] = v;
+    __syncthreads();
+    if (q == 0 && p) {
+        float s = acc[0][o];
+#pragma unroll
+        for (int k = 1; k < RED_SLICES; k++) s += acc[k][o];
+        if (i < n) {
+            const int zi = (int)(i / (27 * 1024)), e = (int)(i % (27 * 1024));
+            const int t = e >> 10, w = (e >> 8) & 3, j = (e >> 6) & 3, l = e & 63;
+            const int co = (zi / (cin / 32)) * 32 + 16 * (w >> 1) + 4 * (l >> 4) + j;
+            const int ci = (zi % (cin / 32)) * 32 + 16 * (w & 1) + (l & 15);
+            dW[((long long)co * 27 + t) * cin + ci] += s;
+        } else {
+            db[i - n] += s;
+        }
+    }
+}
+
 // ------------------------------------------------------------ elementwise
 // dZ = dY * (Y > 0)  (ReLU backward on the stored activation), in place on dY
 __global__ void k_relu_bwd(bf16_t *dY, const bf16_t *Y, long long n) {
@@ -476,6 +717,28 @@ int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n,
     if (cin % 32 != 0 || cout % 32 != 0) return 0;
     Shape s{n, d, h, w};
     const long long P = (long long)n * d * h * w;
+    const bf16_t *z = (const bf16_t *)dZ, *x = (const bf16_t *)X;
+    if (w % 8 == 0 && h % 4 == 0 && d % 4 == 0) {
+        // brick path: about 256 workgroups over (brick chunks) x (32 x 32 blocks)
+        const long long nbricks = P / 128;
+        const int zb = (cout / 32) * (cin / 32);
+        long long per = (nbricks * zb + 255) / 256;
+        if (per < 1) per = 1;
+        const long long nchunks = (nbricks + per - 1) / per;
+        const size_t nw = (size_t)27 * 1024 * (size_t)nchunks * (size_t)zb;
+        float *part = part_buffer(sizeof(float) * (nw + (size_t)nchunks * (size_t)cout));
+        if (!part) {
+            fprintf(stderr, "warm-start conv wgrad: partial-sum buffer allocation failed\n");
+            return 0;
+        }
+        float *bpart = db ? part + nw : nullptr;
+        hipLaunchKernelGGL(k_conv_wgrad_brick, dim3((unsigned)nchunks, 1, (unsigned)zb), dim3(256), 0, 0, z, x, part,
+                           bpart, s, cin, cout, (int)per);
+        const long long m = (long long)zb * 27 * 1024 + (db ? cout : 0);
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((m + 63) / 64)), dim3(256), 0, 0, part, bpart,
+                           (int)nchunks, cin, cout, dW, db);
+        return ws_check("conv wgrad");
+    }
     const int tco = cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
     const int zb = (cout / tco) * (cin / tci);
     // about 2048 workgroups, chunks a multiple of the 32-position slab
@@ -485,7 +748,6 @@ int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n,
     if (chunk < WG_K) chunk = WG_K;
     chunks = (P + chunk - 1) / chunk;
     dim3 grid((unsigned)chunks, 27, (unsigned)zb);
-    const bf16_t *z = (const bf16_t *)dZ, *x = (const bf16_t *)X;
     if (tco == 64 && tci == 64)
         hipLaunchKernelGGL((k_conv_wgrad<64, 64>), grid, dim3(256), 0, 0, z, x, dW, db, s, cin, cout, (int)chunk);
     else if (tco == 64)

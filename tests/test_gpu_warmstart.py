@@ -42,10 +42,15 @@ SHAPES = [  # (n, d, h, w, cin, cout): tails of the 128-position tiles, 1..4 col
 ]
 WGRAD_SHAPES = [  # (n, d, h, w, cin, cout): every 32/64 channel tiling, slab tails
     (2, 3, 5, 7, 32, 32),
-    (1, 4, 4, 8, 64, 64),
     (1, 2, 2, 4, 256, 128),
     (3, 5, 6, 7, 32, 64),
+    (2, 4, 6, 16, 32, 32),
+    # extents divisible into 8 x 4 x 4 bricks: the brick-tiled kernel
+    (1, 4, 4, 8, 64, 64),
     (2, 8, 8, 8, 64, 32),
+    (2, 4, 8, 16, 32, 32),
+    (1, 4, 4, 8, 256, 128),
+    (16, 16, 16, 32, 32, 32),  # the network's first level at batch 16
 ]
 
 
@@ -85,8 +90,11 @@ def test_conv_weight_gradient(gpu, shape):
     G, Gv = dev_bf16(rng.normal(size=(n, d, h, w, cout)))
     dW, db = dev_zeros(cout * 27 * cin, F32), dev_zeros(cout, F32)
     assert L.lbmnn_conv_wgrad(G.ptr, X.ptr, dW.ptr, db.ptr, n, d, h, w, cin, cout)
-    assert rel_l2(dW.download(np.float32, (cout, 27, cin)), C.conv_wgrad(Gv, Xv)) < 1e-6
-    assert rel_l2(db.download(np.float32, cout), Gv.reshape(-1, cout).sum(0)) < 1e-6
+    # exact BF16 products summed in FP32; over 131,072 positions the rounding
+    # of the running sums reaches a few 1e-7
+    tol = 1e-6 if n * d * h * w < 100000 else 2e-6
+    assert rel_l2(dW.download(np.float32, (cout, 27, cin)), C.conv_wgrad(Gv, Xv)) < tol
+    assert rel_l2(db.download(np.float32, cout), Gv.reshape(-1, cout).sum(0)) < tol
 
 
 def test_pool_upsample_relu_bias_mse(gpu):
