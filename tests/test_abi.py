@@ -9,17 +9,19 @@ import tempfile
 
 import pytest
 
-from lattice_fluid_dynamics_amd import lbm, warmstart
+from lattice_fluid_dynamics_amd import lbm, model, warmstart
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_warmstart.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_warmstart.h", "lbm_model.h")]
+# lbm_model.h keeps the reference helpers' own names
+MODEL_NAMES = r"loadOBJ|freeModel|computeModelBounds|createTriangleBuffer|Voxelize_[A-Za-z]+"
 
 
 def declared_functions(headers=HEADERS):
     names = set()
     for h in headers:
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
-        names |= set(re.findall(r"\b((?:LBM|lbmnn)_[A-Za-z0-9_]+)\s*\(", text))
+        names |= set(re.findall(r"\b((?:LBM|lbmnn)_[A-Za-z0-9_]+|" + MODEL_NAMES + r")\s*\(", text))
     return sorted(names)
 
 
@@ -42,7 +44,7 @@ def test_library_exports_every_declared_symbol():
     L = warmstart.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    bound = set(lbm.SIGNATURES) | set(warmstart.WS_SIGNATURES)
+    bound = set(lbm.SIGNATURES) | set(warmstart.WS_SIGNATURES) | set(model.SIGNATURES) | {"LBM_AutoPlaceModel"}
     assert set(declared_functions()) == bound, set(declared_functions()) ^ bound
 
 
@@ -72,6 +74,22 @@ def test_particle_record_is_32_bytes():
         open(c, "w").write(src)
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         assert subprocess.run([exe], capture_output=True, text=True, check=True).stdout == "32"
+
+
+def test_model_records_match_c():
+    """Model, CarBounds, GPUTriangle (48 bytes), ModelPlacement and VoxelGrid:
+    the ctypes mirrors have the C sizes (simulation/lib/model_bounds.h:8-18)."""
+    names = ["Model", "CarBounds", "GPUTriangle", "ModelPlacement", "VoxelGrid", "Vertex", "Face"]
+    src = '#include <stdio.h>\n#include "lbm_model.h"\nint main(void){\n'
+    src += "".join(f'printf("%zu\\n", sizeof({n}));\n' for n in names) + "return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    py = [ctypes.sizeof(getattr(model, n)) if hasattr(model, n) else 48 for n in names]
+    assert out == py and out[2] == 48
 
 
 def test_create_without_device_fails_loudly(monkeypatch):

@@ -1,0 +1,189 @@
+"""Model helpers (include/lbm_model.h): OBJ reading, the caller's placement,
+the triangle buffer LBM_SetSolidMesh takes, and the occupancy voxeliser.
+
+Pinned three ways:
+  * against the reference's own helpers (model_loader.c, model_bounds.c,
+    voxelize.c) compiled into oracle/_ref, on the reference's three real OBJ
+    assets: bit-exact vertices, faces, triangles, bounds, grids and files
+    (where the reference tree and oracle/_ref are present);
+  * against tests/golden/model_meshes.npz, written from those assets by
+    tests/golden/make_model_fixtures.py (its grids come from oracle/_ref);
+  * by closed forms on small hand-written OBJ files.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from lattice_fluid_dynamics_amd import model as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ASSETS = "/root/reference/simulation/assets/3d-files"
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_model.so")
+MESHES = {"car": "car-model.obj", "ahmed25": "ahmed_25deg_m.obj", "ahmed35": "ahmed_35deg_m.obj"}
+GOLDEN = np.load(os.path.join(HERE, "golden", "model_meshes.npz"))
+
+have_ref = pytest.mark.skipif(not (os.path.isdir(ASSETS) and os.path.exists(REF_LIB)),
+                              reason="reference tree or oracle/_ref build absent")
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def ref():
+    return M.bind(ctypes.CDLL(REF_LIB), placement=False)
+
+
+@have_ref
+@pytest.mark.parametrize("key", list(MESHES))
+def test_reader_and_placement_match_reference_build(ref, key):
+    path = os.path.join(ASSETS, MESHES[key])
+    ours, theirs = M.LoadedModel(path), M.LoadedModel(path, lib=ref)
+    assert np.array_equal(bits(ours.vertices()), bits(theirs.vertices()))
+    assert np.array_equal(ours.faces(), theirs.faces())
+    place = M.auto_placement(ours)
+    for p in (place, (1.3, 0.25, -0.5, 0.125, 33.0), (0.07, 0.0, 0.0, 0.0, 360.0)):
+        assert np.array_equal(bits(ours.triangles(*p)), bits(theirs.triangles(*p)))
+        assert ours.bounds(*p) == theirs.bounds(*p)
+    tris, bbox, place2 = M.mesh_from_obj(path)
+    assert place2 == place
+    assert np.array_equal(bits(tris), bits(GOLDEN[key + "_tris"]))
+    assert np.array_equal(np.array(bbox, np.float32), GOLDEN[key + "_bbox"])
+
+
+@have_ref
+@pytest.mark.parametrize("key", list(MESHES))
+@pytest.mark.parametrize("mode", [M.VOXELIZE_MODE_SOLID, M.VOXELIZE_MODE_SURFACE, M.VOXELIZE_MODE_AUTO])
+def test_voxelizer_matches_reference_build(ref, key, mode, tmp_path):
+    path = os.path.join(ASSETS, MESHES[key])
+    ours, theirs = M.LoadedModel(path), M.LoadedModel(path, lib=ref)
+    for R, pad, align in ((32, 0.1, 1), (64, 0.0, 0), (96, 0.25, 1), (64, -1.0, 1), (48, 0.7, 0)):
+        a, b = ours.voxelize(R, pad, align, mode), theirs.voxelize(R, pad, align, mode)
+        assert a.any() and np.array_equal(a, b), (R, pad, align)
+    pa, pb = str(tmp_path / "a.voxbin"), str(tmp_path / "b.voxbin")
+    assert ours.write_voxbin(pa, 40, 0.1, 1, mode) == 0 and theirs.write_voxbin(pb, 40, 0.1, 1, mode) == 0
+    with open(pa, "rb") as fa, open(pb, "rb") as fb:
+        assert fa.read() == fb.read()
+
+
+@pytest.mark.parametrize("key", list(MESHES))
+def test_golden_placement_and_grid(key):
+    """Without the reference tree: the fixture's placed triangles lie in the
+    box the placement promises, and the reference-built 64^3 grid is solid
+    where the mesh is (the fraction the reference produced)."""
+    tris, bbox, place = GOLDEN[key + "_tris"], GOLDEN[key + "_bbox"], GOLDEN[key + "_place"]
+    v = tris.reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
+    assert np.all(tris.reshape(-1, 3, 4)[:, :, 3] == 0)
+    assert np.array_equal(v.min(0), bbox[:3]) and np.array_equal(v.max(0), bbox[3:])
+    ext = bbox[3:] - bbox[:3]
+    assert abs(float(ext.max()) - 2.4) < 1e-5  # largest extent scaled to 2.4
+    assert abs(0.5 * (bbox[0] + bbox[3]) + 2.0) < 1e-5  # centre moved to x = -2
+    assert place[4] in (0.0, 90.0)
+    grid = np.unpackbits(GOLDEN[key + "_vox64"])[:64 ** 3]
+    assert 0.01 < grid.mean() < 0.2
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+CUBE = """# unit cube, quads, v/vt/vn references, a record the reader ignores
+o cube
+v -1 -1 -1
+v  1 -1 -1
+v  1  1 -1
+v -1  1 -1
+v -1 -1  1
+v  1 -1  1
+v  1  1  1
+v -1  1  1
+vt 0 0
+vn 0 0 1
+f 1/1/1 2/1/1 3/1/1
+f 1//1 3//1 4//1
+f 5 6 7
+f 5 7 8
+f 1 2 6
+f 1 6 5
+f 4 3 7
+f 4 7 8
+f 1 4 8
+f 1 8 5
+f 2 3 7 6
+f 2 7 6
+"""
+
+
+def test_reader_on_a_hand_written_file(tmp_path):
+    m = M.LoadedModel(write(tmp_path, "cube.obj", CUBE))
+    v, f = m.vertices(), m.faces()
+    assert v.shape == (8, 3) and f.shape == (12, 3)
+    assert np.array_equal(v[6], [1, 1, 1])
+    assert f[0].tolist() == [1, 2, 3] and f[1].tolist() == [1, 3, 4]
+    assert f[10].tolist() == [2, 3, 7]  # a quad keeps its first triangle
+    tris = m.triangles(1.0, 0.0, 0.0, 0.0, 0.0)
+    assert tris.shape == (12, 12)
+    assert np.array_equal(tris[0], [-1, -1, -1, 0, 1, -1, -1, 0, 1, 1, -1, 0])
+    # 90 degrees about y: (x, z) -> (x c - z s, x s + z c), c = cos(float(pi/2))
+    r = m.triangles(1.0, 0.0, 0.0, 0.0, 90.0)
+    assert np.allclose(r[0, [0, 2]], [1, -1], atol=1e-6)
+
+
+def test_out_of_range_faces_and_missing_files(tmp_path):
+    m = M.LoadedModel(write(tmp_path, "bad.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\nf 1 2 9\nf -1 2 3\n"))
+    assert m.faces().shape == (3, 3)
+    assert m.triangles(1, 0, 0, 0, 0).shape == (1, 12)  # the two bad faces are skipped
+    assert m.voxelize(16) is None  # the voxeliser rejects the model outright
+    empty = M.LoadedModel(str(tmp_path / "missing.obj"))
+    assert empty.m.vertexCount == 0 and empty.m.faceCount == 0
+    assert empty.voxelize(16) is None
+    assert empty.bounds(1, 0, 0, 0, 0) == (-0.5, -0.5, -0.5, 0.5, 0.5, 0.5, 0.0, 0.0, 0.0)
+    assert M.auto_placement(empty) == pytest.approx((1.0, -2.0, -0.1, -0.9, 0.0))
+
+
+def test_placement_closed_forms(tmp_path):
+    """A 4 x 1 x 2 box centred at (10, 20, 30): scale 2.4 / 4, centre to
+    x = -2; the same box longer in z turns 90 degrees."""
+    def box(ex, ey, ez):
+        pts = [(10 + sx * ex / 2, 20 + sy * ey / 2, 30 + sz * ez / 2)
+               for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+        return "".join("v %g %g %g\n" % p for p in pts) + "f 1 2 3\nf 2 3 4\n"
+    m = M.LoadedModel(write(tmp_path, "x.obj", box(4, 1, 2)))
+    s, ox, oy, oz, rot = M.auto_placement(m)
+    assert s == np.float32(2.4) / np.float32(4.0) and rot == 0.0
+    b = m.bounds(s, ox, oy, oz, rot)
+    # single-precision placement of coordinates near 10..30: a few ulp of 20
+    assert b[0] == pytest.approx(-3.2, abs=1e-5) and b[3] == pytest.approx(-0.8, abs=1e-5)
+    assert b[1] == pytest.approx(-0.3, abs=1e-5) and b[4] == pytest.approx(0.3, abs=1e-5)
+    m2 = M.LoadedModel(write(tmp_path, "z.obj", box(2, 1, 4)))
+    s, ox, oy, oz, rot = M.auto_placement(m2)
+    assert rot == 90.0
+    b = m2.bounds(s, ox, oy, oz, rot)
+    assert b[0] == pytest.approx(-3.2, abs=1e-5) and b[3] == pytest.approx(-0.8, abs=1e-5)
+    assert b[2] == pytest.approx(-0.6, abs=1e-5) and b[5] == pytest.approx(0.6, abs=1e-5)
+    s2 = M.auto_placement(m2, scale=0.5)[0]
+    assert s2 == 0.5
+
+
+def test_voxelizer_closed_forms(tmp_path):
+    """The closed cube spans [-(1 - p), 1 - p]^3 after normalisation: solid
+    voxels are those whose centres lie inside, (2 - 2p) R / 2 per axis."""
+    m = M.LoadedModel(write(tmp_path, "cube.obj", CUBE.replace("f 2 3 7 6\nf 2 7 6\n", "f 2 3 7\nf 2 7 6\n")))
+    g = m.voxelize(20, 0.1, 1, M.VOXELIZE_MODE_SOLID)
+    assert g.sum() == 18 ** 3 and g[1:19, 1:19, 1:19].all()
+    # surface mode marks the voxels under each face's bounding box: a shell
+    # at the faces x, y, z = +-0.9, nothing inside it
+    s = m.voxelize(20, 0.1, 1, M.VOXELIZE_MODE_SURFACE)
+    assert s.any() and not s[3:17, 3:17, 3:17].any()
+    assert s[:, :, 1].any() and s[:, :, 18].any()
+    path = str(tmp_path / "g.voxbin")
+    assert m.write_voxbin(path, 20, 0.1, 1, M.VOXELIZE_MODE_SOLID) == 0
+    raw = open(path, "rb").read()
+    assert raw[:4] == b"VOXB" and np.frombuffer(raw[4:12], "<u4").tolist() == [1, 20]
+    assert np.array_equal(np.frombuffer(raw[12:], np.uint8).reshape(20, 20, 20), g)
