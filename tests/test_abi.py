@@ -9,12 +9,12 @@ import tempfile
 
 import pytest
 
-from lattice_fluid_dynamics_amd import lbm, model, warmstart
+from lattice_fluid_dynamics_amd import lbm, model, superres, warmstart
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_warmstart.h", "lbm_model.h")]
-# lbm_model.h keeps the reference helpers' own names
-MODEL_NAMES = r"loadOBJ|freeModel|computeModelBounds|createTriangleBuffer|Voxelize_[A-Za-z]+"
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_warmstart.h", "lbm_model.h", "lbm_superres.h")]
+# lbm_model.h and lbm_superres.h keep the reference helpers' own names
+MODEL_NAMES = r"loadOBJ|freeModel|computeModelBounds|createTriangleBuffer|Voxelize_[A-Za-z]+|SR_[A-Za-z]+"
 
 
 def declared_functions(headers=HEADERS):
@@ -44,7 +44,8 @@ def test_library_exports_every_declared_symbol():
     L = warmstart.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    bound = set(lbm.SIGNATURES) | set(warmstart.WS_SIGNATURES) | set(model.SIGNATURES) | {"LBM_AutoPlaceModel"}
+    bound = (set(lbm.SIGNATURES) | set(warmstart.WS_SIGNATURES) | set(model.SIGNATURES) | {"LBM_AutoPlaceModel"}
+             | set(superres.SR_SIGNATURES))
     assert set(declared_functions()) == bound, set(declared_functions()) ^ bound
 
 
