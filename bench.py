@@ -13,7 +13,9 @@ per second of the whole timed region (inputs resident in HBM; includes the
 boundary prepass, the particle kernel and the host synchronisations).
 `roofline` prices the fused step kernel alone at 152 B/cell/step (19 loads +
 19 stores of FP32) against its average launch time from HIP events recorded on
-the grid's stream around every launch.  `cpu_baseline` times the oracle (the
+the grid's stream around every launch of every 10th frame of the timed region
+(an event pair stalls the queue for several microseconds, so the other frames
+run without them).  `cpu_baseline` times the oracle (the
 C/OpenMP restatement of the reference shaders) on a bounded sample of the same
 workload on the host cores.
 
@@ -76,28 +78,34 @@ def setup_grid(lbm, geometry, w):
     return g, tris, bbox, area
 
 
-def run_frames(lbm, g, parts, w, nsteps, step0, per_launch=True):
+def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
     """nsteps LBM steps in frames of 10 (particles advected after each frame).
-    Returns (wall_s, main_ms_sum, boundary_ms_sum, region_ms_sum)."""
+    With event_every = k > 0, the launches of every k-th frame are bracketed
+    by HIP events on the grid's stream (a recorded event pair idles the GPU
+    for several microseconds, so only a sample of the frames carries them).
+    Returns (wall_s, main_ms_sum, boundary_ms_sum, timed_steps)."""
     nx = w["dims"][0]
     vel_scale = 8.0 / nx
-    main = bnd = region = 0.0
+    main = bnd = 0.0
+    timed = 0
     lbm.LBM_Synchronize(g)
     t0 = time.perf_counter()
     done = 0
     frame = 0
     while done < nsteps:
         n = min(10, nsteps - done)
-        tot, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=per_launch)
-        main += mn
-        bnd += bd
-        region += tot
+        sample = event_every > 0 and frame % event_every == 0
+        _, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=sample)
+        if sample:
+            main += mn
+            bnd += bd
+            timed += n
         done += n
         frame += 1
         if parts is not None:
             lbm.LBM_AdvectParticles(g, parts, float(n), vel_scale, 0, step0 + frame)
     lbm.LBM_Synchronize(g)
-    return time.perf_counter() - t0, main, bnd, region
+    return time.perf_counter() - t0, main, bnd, timed
 
 
 def cpu_baseline(lbm, g, w, steps):
@@ -146,13 +154,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0 = skip)")
-    ap.add_argument("--no-per-launch", action="store_true", help="skip the per-launch kernel events")
+    ap.add_argument("--event-every", type=int, default=10,
+                    help="time the kernel launches of every k-th 10-step frame with HIP events (0 = none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU slab "
               "decomposition is a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
+    # The library prints the reference's setup lines ("LBM Grid: ...") on the
+    # process's stdout; keep stdout for the one JSON line by pointing file
+    # descriptor 1 at stderr until the result is written.
+    sys.stdout.flush()
+    json_out = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
+    out = run(args)
+    sys.stdout.flush()
+    json_out.write(json.dumps(out) + "\n")
+    json_out.flush()
 
+
+def run(args):
     from lattice_fluid_dynamics_amd import geometry, lbm
 
     w = WORKLOADS[args.workload]
@@ -167,9 +188,9 @@ def main():
     tuned_block, tune_ms = lbm.LBM_Autotune(g, 10, w["u"], 0.0, 0.0)
     log(f"autotune: workgroup {tuned_block} (ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
     if args.warmup:
-        run_frames(lbm, g, parts, w, args.warmup, 0, per_launch=False)
-    wall, main_ms, bnd_ms, region_ms = run_frames(lbm, g, parts, w, args.steps, 10 ** 6,
-                                                   per_launch=not args.no_per_launch)
+        run_frames(lbm, g, parts, w, args.warmup, 0)
+    wall, main_ms, bnd_ms, timed_steps = run_frames(lbm, g, parts, w, args.steps, 10 ** 6,
+                                                    event_every=args.event_every)
     mlups = cells * args.steps / wall / 1e6
     fx, fy, fz, px, py, pz = lbm.LBM_ComputeDragForceDecomposed(g)
     dyn = 0.5 * w["u"] ** 2 * area
@@ -181,15 +202,15 @@ def main():
     bpc = BYTES_PER_CELL[w["storage"]]
     roof = None
     if main_ms > 0:
-        avg_s = main_ms / args.steps / 1e3
+        avg_s = main_ms / timed_steps / 1e3
         achieved = bpc * cells / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload),
                 "kernel": "k_step (fused stream+collide)", "bytes_per_cell": bpc,
-                "avg_launch_ms": round(main_ms / args.steps, 5),
+                "avg_launch_ms": round(main_ms / timed_steps, 5), "timed_launches": timed_steps,
                 "copy_kernel_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
                 "workgroup": tuned_block,
-                "boundary_prepass_avg_ms": round(bnd_ms / max(args.steps, 1), 5)}
+                "boundary_prepass_avg_ms": round(bnd_ms / max(timed_steps, 1), 5)}
     cpu = None
     if args.cpu_steps > 0:
         log("timing the CPU oracle ...")
@@ -211,7 +232,7 @@ def main():
         "cd": round(abs(fx) / dyn, 4) if dyn > 0 else None,
         "hbm_roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpc / 1e6, 1),
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
