@@ -157,7 +157,11 @@ LBMGrid *LBM_Create(int sizeX, int sizeY, int sizeZ, float viscosity) {
     im->geomDirty = 1;
 
     char err[256] = {0};
-    im->dev = lbmdev_create(sizeX, sizeY, sizeZ, g->populationStorage, err, sizeof(err));
+    /* Grids beyond 2^30 cells cannot hold FP32 populations (32-bit slot
+     * offsets): their buffers start in FP16S, and LBM_InitializeFlow fails
+     * unless populationStorage is set to a 16-bit or FP64 format first. */
+    const int initial = cells > (1LL << 30) ? LBM_STORAGE_FP16 : g->populationStorage;
+    im->dev = lbmdev_create(sizeX, sizeY, sizeZ, initial, err, sizeof(err));
     if (!im->dev) {
         char msg[400];
         snprintf(msg, sizeof(msg), "LBM_Create: %s", err);
@@ -546,15 +550,17 @@ static int prepare_flow(LBMGrid *g) {
     Impl *im = impl_of(g);
     if (g->populationStorage != lbmdev_storage(im->dev)) {
         char err[256] = {0};
+        const int previous = lbmdev_storage(im->dev);
         lbmdev_destroy(im->dev);
         im->dev = lbmdev_create(g->sizeX, g->sizeY, g->sizeZ, g->populationStorage, err, sizeof(err));
         if (!im->dev) {
             char msg[400];
             snprintf(msg, sizeof(msg), "LBM_InitializeFlow: %s", err);
             note_error(msg);
-            /* fall back to a usable FP32 grid so later calls stay valid */
-            g->populationStorage = LBM_STORAGE_FP32;
-            im->dev = lbmdev_create(g->sizeX, g->sizeY, g->sizeZ, LBM_STORAGE_FP32, err, sizeof(err));
+            /* fall back to the format the grid had, so later calls stay valid */
+            char err2[256] = {0};
+            g->populationStorage = previous;
+            im->dev = lbmdev_create(g->sizeX, g->sizeY, g->sizeZ, previous, err2, sizeof(err2));
             if (im->dev) {
                 g->velocityBuffer = lbmdev_velocity_ptr(im->dev);
                 g->solidBuffer = lbmdev_solid_ptr(im->dev);

@@ -1155,6 +1155,15 @@ extern "C" LbmDev *lbmdev_create(int nx, int ny, int nz, int storage, char *err,
         free(d);
         return NULL;
     }
+    // 32- and 16-bit slots are addressed by 32-bit byte offsets within a plane
+    if (storage != LBMDEV_FP64 && (unsigned long long)nc * elem_size(storage) > (1ULL << 32)) {
+        snprintf(err, errlen,
+                 "grid %dx%dx%d has %lld cells; FP32 populations address at most 2^30 cells per grid "
+                 "(32-bit slot offsets): use FP16/BF16 or FP64 storage",
+                 nx, ny, nz, nc);
+        free(d);
+        return NULL;
+    }
     d->ncell = (int)nc;
     d->fmt = storage;
     d->elem = elem_size(storage);
@@ -1838,7 +1847,8 @@ extern "C" LbmBatchDev *lbmbatch_create(int storage, int nt, const int *dims, co
     for (int t = 0; t < nt; t++) {
         const int nx = dims[3 * t], ny = dims[3 * t + 1], nz = dims[3 * t + 2];
         const long long nc = (long long)nx * ny * nz;
-        if (nx < 2 || ny < 1 || nz < 1 || cells + nc > 0x7fffffffLL - 256) {
+        if (nx < 2 || ny < 1 || nz < 1 || cells + nc > 0x7fffffffLL - 256 ||
+            (storage != LBMDEV_FP64 && (unsigned long long)nc * b->elem > (1ULL << 32))) {
             snprintf(err, errlen, "tunnel %d (%dx%dx%d) is outside the supported range", t, nx, ny, nz);
             lbmbatch_destroy(b);
             return NULL;

@@ -80,3 +80,30 @@ def test_uniform_inflow_is_a_fixed_point_at_full_size(gpu, regularized):
     assert np.max(np.abs(vel[:, 0] - U)) < 2e-6
     assert np.max(np.abs(vel[:, 1:3])) < 1e-7
     assert np.max(np.abs(vel[:, 3] - 1.0)) < 2e-6
+
+
+def test_grid_beyond_2_30_cells(gpu):
+    """1024 x 1024 x 1025 cells (2^30 + 2^20).  FP32 populations would need
+    slot byte offsets past 2^32 within a plane: LBM_InitializeFlow refuses
+    FP32 storage with an error and leaves the grid usable.  In FP16 storage
+    (offsets up to 2^31) the lattice runs.  With periodic y/z a uniform inflow
+    evolves identically in every x-column whatever ny and nz are, so the mass
+    per cell equals that of a 1024 x 4 x 5 grid to FP64 summation rounding
+    (FP16 storage itself shifts both by the same 1.7e-5)."""
+    def run(ny, nz, expect_fp32_refused):
+        g = lbm.LBM_Create(1024, ny, nz, 0.1 / 3)
+        g.tau = 0.6
+        g.periodicYZ = 1
+        if expect_fp32_refused:
+            assert lbm.LBM_InitializeFlow(g, U, 0, 0) == 0
+            assert "2^30" in lbm.last_error()
+        g.populationStorage = lbm.LBM_STORAGE_FP16
+        assert lbm.LBM_InitializeFlow(g, U, 0, 0) == 1
+        lbm.LBM_StepN(g, 6, U, 0, 0)
+        m = lbm.LBM_ComputeTotalMass(g) / (1024 * ny * nz)
+        lbm.LBM_Free(g)
+        return m
+    small = run(4, 5, False)
+    large = run(1024, 1025, True)
+    print("mass per cell: 1024x4x5 %.15f, 1024x1024x1025 %.15f" % (small, large))
+    assert abs(large - small) <= 1e-10 and abs(large - 1.0) < 1e-4
