@@ -394,6 +394,30 @@ def test_fp64_algorithm_exact_all_boundaries(gpu, case, periodic, op, smag):
     assert np.allclose(F[:3], ex_o[:3], rtol=1e-8 if smag else 1e-9, atol=1e-12)
 
 
+def _no_body(g, og):
+    pass
+
+
+@pytest.mark.parametrize("dims,body", [((37, 19, 11), True), ((65, 7, 9), True), ((130, 3, 2), False),
+                                       ((3, 3, 3), False), ((2, 5, 3), False)])
+@pytest.mark.parametrize("periodic", [1, 0])
+@pytest.mark.parametrize("op", ["bgk", "mrt", "reg"])
+def test_ragged_grids_match_oracle(gpu, dims, body, periodic, op):
+    """Odd and prime extents (the last workgroup partly past the lattice, no
+    row-aligned stores), single-cell-thick y/z extents and nx = 2 (the -x
+    and +x neighbours are the same cell, and every cell is an inlet or an
+    outlet cell): steps 1-6 against the FP32 oracle."""
+    setup = setup_sphere(-0.6, 0.1, -0.1, 0.9) if body else _no_body
+    g, og = make_pair(*dims, 0.02, setup, tau=0.58, periodicYZ=periodic, useMRT=int(op == "mrt"),
+                      useRegularized=int(op == "reg"), useSmagorinsky=1, smagorinskyCs=0.1)
+    for n in (1, 2, 3):
+        gpu.LBM_StepN(g, n, 0.05, 0, 0)
+        og.step(0.05, 0, 0, n=n)
+        d = compare(g, og)
+        assert d["f"] < 1e-6 and d["f_new"] < 1e-6 and d["u"] < 1e-5 and d["rho"] < 1e-6, (n, d)
+    print(dims, op, periodic, d)
+
+
 @pytest.mark.parametrize("op", ["reg", "mrt"])
 def test_row_aligned_phase_b_is_bitwise_identical(gpu, op):
     """Workgroup 512 on a 512-cell row takes the row-aligned phase-B stores
