@@ -131,6 +131,14 @@ void lbm_note_error(const char *msg);
 /* warmstart_kernels.hip: device helpers of the warm-start network */
 int lbmnn_memset(void *p, int value, unsigned long long bytes);
 int lbmnn_synchronize(void);
+/* Zeroes n float buffers (n <= LBMNN_ZERO_MAX) with one asynchronous launch. */
+#define LBMNN_ZERO_MAX 40
+int lbmnn_zero_list(float *const *ptrs, const long long *counts, int n);
+/* Pinned host memory (hipHostMalloc) and an asynchronous device-to-host
+ * copy ordered after the work already queued by the network. */
+void *lbmnn_host_alloc(size_t bytes);
+void lbmnn_host_free(void *p);
+int lbmnn_copy_to_host_async(void *host, const void *dev, size_t bytes);
 
 #ifdef __cplusplus
 }

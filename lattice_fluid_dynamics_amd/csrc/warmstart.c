@@ -48,7 +48,8 @@ struct LBMWarmStart {
     Conv conv[NCONV];
     void *act[NACT], *grad[NACT], *tmp[3];
     float *yf, *loss;
-    void *in1;      /* one packed input (inference) */
+    float *lossHost; /* pinned: the loss of the last step, copied asynchronously */
+    void *in1;       /* one packed input (inference) */
     float *out1;    /* one packed output */
     int step;
 };
@@ -105,6 +106,7 @@ void LBM_WarmStartFree(LBMWarmStart *ws) {
         if (ws->tmp[l]) lbmdev_free(ws->tmp[l]);
     if (ws->yf) lbmdev_free(ws->yf);
     if (ws->loss) lbmdev_free(ws->loss);
+    lbmnn_host_free(ws->lossHost);
     if (ws->in1) lbmdev_free(ws->in1);
     if (ws->out1) lbmdev_free(ws->out1);
     free(ws);
@@ -175,6 +177,12 @@ LBMWarmStart *LBM_WarmStartCreate(int cnx, int cny, int cnz, int maxBatch, unsig
     for (int l = 0; l < 3; l++) DALLOC(ws->tmp[l], 2 * (size_t)positions(ws, l, maxBatch) * (32 << l));
     DALLOC(ws->yf, sizeof(float) * (size_t)positions(ws, 0, maxBatch) * 32);
     DALLOC(ws->loss, sizeof(float));
+    ws->lossHost = (float *)lbmnn_host_alloc(sizeof(float));
+    if (!ws->lossHost) {
+        ws_err("warm start: pinned host allocation failed");
+        LBM_WarmStartFree(ws);
+        return NULL;
+    }
     DALLOC(ws->in1, 2 * (size_t)positions(ws, 0, 1) * 32);
     DALLOC(ws->out1, sizeof(float) * (size_t)positions(ws, 0, 1) * 32);
     if (!lbmnn_synchronize()) {
@@ -276,36 +284,50 @@ static int backward(LBMWarmStart *ws, const void *x0, int B) {
     return ok;
 }
 
+/* Queues gradient zeroing, forward, loss and backward, and the loss copy
+ * into pinned host memory; nothing here waits for the device, so the host
+ * runs ahead of the kernels (a blocking memset per buffer drained the queue
+ * 31 times a step). */
+static int queue_gradients(LBMWarmStart *ws, const void *input, const float *target, int batch) {
+    float *zp[2 * NCONV + 1];
+    long long zn[2 * NCONV + 1];
+    for (int k = 0; k < NCONV; k++) {
+        zp[2 * k] = ws->conv[k].gw;
+        zn[2 * k] = ws->conv[k].nw;
+        zp[2 * k + 1] = ws->conv[k].gb;
+        zn[2 * k + 1] = COUT[k];
+    }
+    zp[2 * NCONV] = ws->loss;
+    zn[2 * NCONV] = 1;
+    const long long n = positions(ws, 0, batch) * 32;
+    int ok = lbmnn_zero_list(zp, zn, 2 * NCONV + 1);
+    ok = ok && forward(ws, input, batch, ws->yf);
+    ok = ok && lbmnn_mse(ws->yf, target, ws->grad[A_Y], ws->loss, n);
+    ok = ok && backward(ws, input, batch);
+    ok = ok && lbmnn_copy_to_host_async(ws->lossHost, ws->loss, sizeof(float));
+    return ok;
+}
+
 float LBM_WarmStartGradients(LBMWarmStart *ws, const void *input, const float *target, int batch) {
     if (!ws || !input || !target || batch <= 0 || batch > ws->maxBatch) {
         ws_err("LBM_WarmStartGradients: bad arguments (batch must be 1..maxBatch)");
         return -1.0f;
     }
-    int ok = 1;
-    for (int k = 0; k < NCONV && ok; k++) {
-        ok = lbmnn_memset(ws->conv[k].gw, 0, sizeof(float) * (size_t)ws->conv[k].nw) &&
-             lbmnn_memset(ws->conv[k].gb, 0, sizeof(float) * (size_t)COUT[k]);
-    }
-    const long long n = positions(ws, 0, batch) * 32;
-    ok = ok && lbmnn_memset(ws->loss, 0, sizeof(float));
-    ok = ok && forward(ws, input, batch, ws->yf);
-    ok = ok && lbmnn_mse(ws->yf, target, ws->grad[A_Y], ws->loss, n);
-    ok = ok && backward(ws, input, batch);
-    float loss = -1.0f;
-    ok = ok && lbmnn_synchronize() && lbmdev_memcpy_d2h(&loss, ws->loss, sizeof(float));
-    if (!ok) {
+    if (!queue_gradients(ws, input, target, batch) || !lbmnn_synchronize()) {
         ws_err("LBM_WarmStartGradients: device error");
         return -1.0f;
     }
-    return loss;
+    return *ws->lossHost;
 }
 
 float LBM_WarmStartTrainStep(LBMWarmStart *ws, const void *input, const float *target, int batch, float lr,
                              float weightDecay) {
-    const float loss = LBM_WarmStartGradients(ws, input, target, batch);
-    if (loss < 0.0f) return loss;
+    if (!ws || !input || !target || batch <= 0 || batch > ws->maxBatch) {
+        ws_err("LBM_WarmStartTrainStep: bad arguments (batch must be 1..maxBatch)");
+        return -1.0f;
+    }
+    int ok = queue_gradients(ws, input, target, batch);
     ws->step++;
-    int ok = 1;
     for (int k = 0; k < NCONV && ok; k++) {
         Conv *c = &ws->conv[k];
         ok = lbmnn_adamw(c->w, c->wb, c->gw, c->mw, c->vw, c->nw, lr, 0.9f, 0.999f, 1e-8f, weightDecay, ws->step) &&
@@ -313,10 +335,10 @@ float LBM_WarmStartTrainStep(LBMWarmStart *ws, const void *input, const float *t
              lbmnn_conv_mirror(c->wb, c->wm, CIN[k], COUT[k]);
     }
     if (!ok || !lbmnn_synchronize()) {
-        ws_err("LBM_WarmStartTrainStep: device error in the update");
+        ws_err("LBM_WarmStartTrainStep: device error");
         return -1.0f;
     }
-    return loss;
+    return *ws->lossHost;
 }
 
 int LBM_WarmStartForward(LBMWarmStart *ws, const void *input, int batch, float *output) {

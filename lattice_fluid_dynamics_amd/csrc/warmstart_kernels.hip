@@ -18,6 +18,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include "lbm_device.h"
+
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef uint16_t bf16_t;  // storage
@@ -129,6 +131,84 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, 
                 if (Yf) Yf[(long long)p * cout + co] = v;
             }
         }
+}
+
+// Brick-tiled forward for extents that divide into 8 x 4 x 4 bricks and
+// channel counts that are multiples of 32: one workgroup per (brick, 32
+// output channels).  Per 32-channel input chunk the brick's 10 x 6 x 6 input
+// halo is staged once in LDS (channels-last rows padded to 40 bf16, so the
+// 16 rows a wave reads start on distinct bank groups); wave w then runs the
+// 32 positions of brick plane z = w (M = 2 x 16: x fastest, two y rows per
+// block) against the 32 output channels (N = 2 x 16) through all 27 taps:
+// per tap two A reads from the halo (16 bytes: 8 channels of the tap-shifted
+// position) and two B reads of the weight rows (L1/L2-resident), four MFMAs.
+// The input is read once per brick plus halo instead of once per tap.
+constexpr int FB_LD = 40;    // bf16 per staged halo position
+constexpr int FB_HALO = 360; // 10 x 6 x 6
+
+__global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
+                                                        const float *__restrict__ bias, bf16_t *__restrict__ Y,
+                                                        float *__restrict__ Yf, Shape s, int cin, int cout, int act) {
+    __shared__ __align__(16) bf16_t halo[FB_HALO * FB_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    const int co0 = blockIdx.y * 32;
+    const int bw = s.w / 8, bh = s.h / 4, bps = bw * bh * (s.d / 4);
+    const int sn = blockIdx.x / bps, loc = blockIdx.x % bps;
+    const int x0 = (loc % bw) * 8, y0 = ((loc / bw) % bh) * 4, z0 = (loc / (bw * bh)) * 4;
+    const long long sbase = (long long)sn * s.d * s.h * s.w;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    for (int c0 = 0; c0 < cin; c0 += 32) {
+        for (int e = tid; e < FB_HALO * 4; e += 256) {
+            const int hp = e >> 2, cg = e & 3;
+            const int X_ = x0 + hp % 10 - 1, Y_ = y0 + (hp / 10) % 6 - 1, Z_ = z0 + hp / 60 - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (X_ >= 0 && X_ < s.w && Y_ >= 0 && Y_ < s.h && Z_ >= 0 && Z_ < s.d)
+                v = *(const uint4 *)(X + (sbase + ((long long)Z_ * s.h + Y_) * s.w + X_) * cin + c0 + 8 * cg);
+            *(uint4 *)(halo + hp * FB_LD + 8 * cg) = v;
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int t = 0; t < 27; t++) {
+            const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets (tap + 1)
+            Pack8 a[2], b[2];
+#pragma unroll
+            for (int mb = 0; mb < 2; mb++) {
+                const int hp = ((wave + dz) * 6 + 2 * mb + (r >> 3) + dy) * 10 + (r & 7) + dx;
+                a[mb].u = *(const uint4 *)(halo + hp * FB_LD + 8 * kq);
+            }
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                b[nb].u = *(const uint4 *)(Wt + ((long long)(co0 + 16 * nb + r) * 27 + t) * cin + c0 + 8 * kq);
+#pragma unroll
+            for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb].v, b[nb].v, acc[mb][nb], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D[row = position 4 kq + j][col = channel r]
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int co = co0 + 16 * nb + r;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = 4 * kq + j;
+                const long long p = sbase + ((long long)(z0 + wave) * s.h + y0 + 2 * mb + (m >> 3)) * s.w + x0 + (m & 7);
+                float v = acc[mb][nb][j] + bv;
+                if (act == 1) v = fmaxf(v, 0.0f);
+                Y[p * cout + co] = f2bf(v);
+                if (Yf) Yf[p * cout + co] = v;
+            }
+    }
 }
 
 // Sum of the split-K partials in slice order (deterministic), bias, activation.
@@ -508,6 +588,21 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ 
     }
 }
 
+// Zero-fill of up to LBMNN_ZERO_MAX float buffers in one launch: blockIdx.y
+// selects the buffer, the x blocks stride over it.
+struct ZeroList {
+    float *p[LBMNN_ZERO_MAX];
+    long long count[LBMNN_ZERO_MAX];
+    int n;
+};
+
+__global__ void k_zero_list(ZeroList z) {
+    float *p = z.p[blockIdx.y];
+    const long long n = z.count[blockIdx.y];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
 // ------------------------------------------------------------ elementwise
 // dZ = dY * (Y > 0)  (ReLU backward on the stored activation), in place on dY
 __global__ void k_relu_bwd(bf16_t *dY, const bf16_t *Y, long long n) {
@@ -674,6 +769,13 @@ int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, flo
     if (cin % 32 != 0 || cout % 16 != 0) return 0;
     Shape s{n, d, h, w};
     const long long P = (long long)n * d * h * w;
+    // brick path: enough (brick, 32-channel) workgroups to fill the chip
+    if (w % 8 == 0 && h % 4 == 0 && d % 4 == 0 && cout % 32 == 0 && (P / 128) * (cout / 32) >= 256) {
+        dim3 grid((unsigned)(P / 128), (unsigned)(cout / 32));
+        hipLaunchKernelGGL(k_conv_fwd_brick, grid, dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)W, bias,
+                           (bf16_t *)Y, Yf, s, cin, cout, relu);
+        return ws_check("conv forward");
+    }
     const int nbTotal = cout / 16;
     const int NB = nbTotal >= 4 ? 4 : nbTotal;
     const long long base = ((P + 127) / 128) * ((nbTotal + NB - 1) / NB);
@@ -823,6 +925,31 @@ int lbmnn_f32_to_bf16(const float *a, void *b, long long n) {
 
 int lbmnn_memset(void *p, int value, unsigned long long bytes) {
     return hipMemset(p, value, bytes) == hipSuccess;
+}
+
+int lbmnn_zero_list(float *const *ptrs, const long long *counts, int n) {
+    if (n <= 0 || n > LBMNN_ZERO_MAX) return 0;
+    ZeroList z;
+    z.n = n;
+    for (int i = 0; i < n; i++) {
+        z.p[i] = ptrs[i];
+        z.count[i] = counts[i];
+    }
+    hipLaunchKernelGGL(k_zero_list, dim3(64, (unsigned)n), dim3(256), 0, 0, z);
+    return ws_check("zero");
+}
+
+void *lbmnn_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    return hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+}
+
+void lbmnn_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
+
+int lbmnn_copy_to_host_async(void *host, const void *dev, size_t bytes) {
+    return hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, 0) == hipSuccess;
 }
 
 int lbmnn_synchronize(void) { return hipDeviceSynchronize() == hipSuccess; }

@@ -33,8 +33,10 @@ def dev_zeros(n, itemsize):
 
 
 SHAPES = [  # (n, d, h, w, cin, cout): tails of the 128-position tiles, 1..4 column blocks;
-    # small grids split the taps over workgroup slices, the first shape does not
+    # small grids split the taps over workgroup slices; the first two divide into
+    # 8 x 4 x 4 bricks with enough workgroups for the brick-tiled kernel
     (4, 16, 32, 32, 32, 32),
+    (8, 8, 16, 32, 64, 96),
     (2, 3, 5, 7, 32, 48),
     (1, 4, 4, 8, 64, 64),
     (1, 2, 2, 4, 256, 128),
