@@ -216,8 +216,17 @@ __global__ void k_conv_epilogue(const float *__restrict__ part, int nsplit, cons
                                 bf16_t *__restrict__ Y, float *__restrict__ Yf, long long n, int cout, int act) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // slices added in order; the loads of 3 slices are issued before their adds
     float v = 0.0f;
-    for (int z = 0; z < nsplit; z++) v += part[(long long)z * n + i];
+    int z = 0;
+    for (; z + 3 <= nsplit; z += 3) {
+        const float a = part[(long long)z * n + i], b = part[(long long)(z + 1) * n + i],
+                    c = part[(long long)(z + 2) * n + i];
+        v += a;
+        v += b;
+        v += c;
+    }
+    for (; z < nsplit; z++) v += part[(long long)z * n + i];
     if (bias) v += bias[i % cout];
     if (act == 1) v = fmaxf(v, 0.0f);
     Y[i] = f2bf(v);
