@@ -143,12 +143,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, 
 // per tap two A reads from the halo (16 bytes: 8 channels of the tap-shifted
 // position) and two B reads of the weight rows (L1/L2-resident), four MFMAs.
 // The input is read once per brick plus halo instead of once per tap.
+// Small grids split the taps over gridDim.z slices (as k_conv_fwd does):
+// slice z writes its FP32 partial sum to part[z][p][co] and k_conv_epilogue
+// finishes the result.
 constexpr int FB_LD = 40;    // bf16 per staged halo position
 constexpr int FB_HALO = 360; // 10 x 6 x 6
 
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
                                                         const float *__restrict__ bias, bf16_t *__restrict__ Y,
-                                                        float *__restrict__ Yf, Shape s, int cin, int cout, int act) {
+                                                        float *__restrict__ Yf, float *__restrict__ part, Shape s,
+                                                        int cin, int cout, int act) {
     __shared__ __align__(16) bf16_t halo[FB_HALO * FB_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, kq = lane >> 4;
@@ -172,8 +177,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
             *(uint4 *)(halo + hp * FB_LD + 8 * cg) = v;
         }
         __syncthreads();
-#pragma unroll 3
-        for (int t = 0; t < 27; t++) {
+        // one tap: two A reads from the halo, two B reads of weight rows, four MFMAs
+        auto tap = [&](int t) {
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets (tap + 1)
             Pack8 a[2], b[2];
 #pragma unroll
@@ -189,6 +194,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
 #pragma unroll
                 for (int nb = 0; nb < 2; nb++)
                     acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb].v, b[nb].v, acc[mb][nb], 0, 0, 0);
+        };
+        if (SPLIT) {
+            for (int t = blockIdx.z * 27 / gridDim.z; t < (int)((blockIdx.z + 1) * 27 / gridDim.z); t++) tap(t);
+        } else {
+#pragma unroll 3
+            for (int t = 0; t < 27; t++) tap(t);
         }
         __syncthreads();
     }
@@ -203,6 +214,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
             for (int j = 0; j < 4; j++) {
                 const int m = 4 * kq + j;
                 const long long p = sbase + ((long long)(z0 + wave) * s.h + y0 + 2 * mb + (m >> 3)) * s.w + x0 + (m & 7);
+                if (SPLIT) {
+                    const long long P = (long long)s.n * s.d * s.h * s.w;
+                    part[((long long)blockIdx.z * P + p) * cout + co] = acc[mb][nb][j];
+                    continue;
+                }
                 float v = acc[mb][nb][j] + bv;
                 if (act == 1) v = fmaxf(v, 0.0f);
                 Y[p * cout + co] = f2bf(v);
@@ -778,11 +794,34 @@ int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, flo
     if (cin % 32 != 0 || cout % 16 != 0) return 0;
     Shape s{n, d, h, w};
     const long long P = (long long)n * d * h * w;
-    // brick path: enough (brick, 32-channel) workgroups to fill the chip
-    if (w % 8 == 0 && h % 4 == 0 && d % 4 == 0 && cout % 32 == 0 && (P / 128) * (cout / 32) >= 256) {
-        dim3 grid((unsigned)(P / 128), (unsigned)(cout / 32));
-        hipLaunchKernelGGL(k_conv_fwd_brick, grid, dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)W, bias,
-                           (bf16_t *)Y, Yf, s, cin, cout, relu);
+    // brick path; the taps split over 3, 9 or 27 slices while the
+    // (brick, 32-channel) workgroups are fewer than 128 (measured at the
+    // network's shapes: splitting 128 or more workgroups only adds partial
+    // traffic, below that the slices fill the chip)
+    if (w % 8 == 0 && h % 4 == 0 && d % 4 == 0 && cout % 32 == 0) {
+        const long long wgs = (P / 128) * (cout / 32);
+        int nsplit = 1;
+        while (nsplit < 27 && wgs * nsplit < 128) nsplit *= 3;
+        float *part = nullptr;
+        if (nsplit > 1) {
+            part = part_buffer(sizeof(float) * (size_t)nsplit * (size_t)P * (size_t)cout);
+            if (!part) {
+                fprintf(stderr, "warm-start conv forward: split-K buffer allocation failed\n");
+                return 0;
+            }
+        }
+        dim3 grid((unsigned)(P / 128), (unsigned)(cout / 32), (unsigned)nsplit);
+        if (part)
+            hipLaunchKernelGGL(k_conv_fwd_brick<true>, grid, dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)W,
+                               bias, (bf16_t *)Y, Yf, part, s, cin, cout, relu);
+        else
+            hipLaunchKernelGGL(k_conv_fwd_brick<false>, grid, dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)W,
+                               bias, (bf16_t *)Y, Yf, part, s, cin, cout, relu);
+        if (part) {
+            const long long m = P * cout;
+            hipLaunchKernelGGL(k_conv_epilogue, g1(m), dim3(256), 0, 0, part, nsplit, bias, (bf16_t *)Y, Yf, m, cout,
+                               relu);
+        }
         return ws_check("conv forward");
     }
     const int nbTotal = cout / 16;
