@@ -9,12 +9,14 @@
 //   2. skip: the shader's trilinear skip of each of the 8 sub-cells from the
 //      centre and its +x/+y/+z neighbours, into LDS skip[cell][32];
 //   3. four layers 108(112) -> 128 -> 128 -> 128 -> 32: per 16-wide K chunk
-//      a lane loads a float4 of its activation row (LDS) and a float4 of its
-//      weight row (global, L2-resident), and the 4 K steps of the MFMA use
-//      the 4 components.  This permutes K inside each chunk identically for
-//      A and B, so both operands come from row-major layouts.  The bias seeds
-//      the accumulators; ReLU follows layers 0-2; each wave writes its
-//      outputs back over its own activation rows (no other wave reads them);
+//      the workgroup stages the chunk's weight rows in LDS once (the next
+//      chunk's loads in flight meanwhile), and a lane reads a
+//      float4 of its activation row and a float4 of its weight row; the 4 K
+//      steps of the MFMA use the 4 components.  This permutes K inside each
+//      chunk identically for A and B, so both operands come from row-major
+//      layouts.  The bias seeds the accumulators; ReLU follows layers 0-2;
+//      each wave writes its outputs back over its own activation rows (no
+//      other wave reads them);
 //   4. the last layer's 32 outputs + skip are de-normalised (v * std + mean)
 //      and stored to the 2x2x2 fine cells.
 // Arithmetic per coarse cell: 2 (108 128 + 2 128 128 + 128 32) = 101,376
@@ -49,27 +51,74 @@ struct SrArgs {
     int fx, fy;
 };
 
-// acc[nb] = bias + act(16 rows) x W^T for output columns 16 nb .. 16 nb + 15
+// Weight rows of one 16-wide K chunk, staged in LDS for all four waves:
+// Bs[n][k] with rows padded to 20 floats (16-byte aligned, and the 16 rows a
+// wave reads start on distinct bank groups).
+constexpr int BS_LD = 20;
+
+// Chunk c of W (N rows of K floats) through registers: the N x 4 float4
+// grid (row n = e / 4, column quad e % 4) has 512 cells for N = 128 (two
+// per thread) and 128 for N = 32 (threads 0..127, one each).
+struct ChunkRegs {
+    float4 v0, v1;
+};
+
+template <int N>
+__device__ __forceinline__ void fetch_chunk(const float *__restrict__ W, int K, int c, int tid, ChunkRegs &v) {
+    if (N == 128) {
+        v.v0 = *(const float4 *)(W + (tid >> 2) * K + 16 * c + 4 * (tid & 3));
+        v.v1 = *(const float4 *)(W + ((tid + 256) >> 2) * K + 16 * c + 4 * (tid & 3));
+    } else if (tid < N * 4) {
+        v.v0 = *(const float4 *)(W + (tid >> 2) * K + 16 * c + 4 * (tid & 3));
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void put_chunk(float *Bs, int tid, const ChunkRegs &v) {
+    if (N == 128) {
+        *(float4 *)(Bs + (tid >> 2) * BS_LD + 4 * (tid & 3)) = v.v0;
+        *(float4 *)(Bs + ((tid + 256) >> 2) * BS_LD + 4 * (tid & 3)) = v.v1;
+    } else if (tid < N * 4) {
+        *(float4 *)(Bs + (tid >> 2) * BS_LD + 4 * (tid & 3)) = v.v0;
+    }
+}
+
+// acc[nb] = bias + act(16 rows) x W^T for output columns 16 nb .. 16 nb + 15.
+// The chunks of W pass through the buffer Bs[128 * BS_LD]: while the waves
+// multiply chunk c, the next chunk's global loads are in flight; it is stored
+// after a barrier.  One buffer (52 KB of LDS per workgroup, three workgroups
+// per CU) measured faster than two (62 KB, two per CU).  Every wave of the
+// workgroup calls this with the same K and NB (barriers).
 template <int K, int NB>
 __device__ __forceinline__ void mlp_layer(const float *rows, const float *__restrict__ W, const float *__restrict__ b,
-                                          int r, int q, f32x4 (&acc)[NB]) {
+                                          float *Bs, int tid, int r, int q, f32x4 (&acc)[NB]) {
+    constexpr int N = 16 * NB, NC = K / 16;
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
         const float bv = b[16 * nb + r];
         acc[nb] = f32x4{bv, bv, bv, bv};
     }
-#pragma unroll 2
-    for (int c = 0; c < K / 16; c++) {
+    ChunkRegs v;
+    v.v0 = v.v1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    fetch_chunk<N>(W, K, 0, tid, v);
+    put_chunk<N>(Bs, tid, v);
+    __syncthreads();
+    for (int c = 0; c < NC; c++) {
+        float *cur = Bs;
+        if (c + 1 < NC) fetch_chunk<N>(W, K, c + 1, tid, v);
         const float4 a = *(const float4 *)(rows + r * LD + 16 * c + 4 * q);
-        float4 w[NB];
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++) w[nb] = *(const float4 *)(W + (16 * nb + r) * K + 16 * c + 4 * q);
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) {
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, w[nb].x, acc[nb], 0, 0, 0);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, w[nb].y, acc[nb], 0, 0, 0);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, w[nb].z, acc[nb], 0, 0, 0);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, w[nb].w, acc[nb], 0, 0, 0);
+            const float4 w = *(const float4 *)(cur + (16 * nb + r) * BS_LD + 4 * q);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, w.x, acc[nb], 0, 0, 0);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, w.y, acc[nb], 0, 0, 0);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, w.z, acc[nb], 0, 0, 0);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, w.w, acc[nb], 0, 0, 0);
+        }
+        __syncthreads();
+        if (c + 1 < NC) {
+            put_chunk<N>(Bs, tid, v);
+            __syncthreads();
         }
     }
 }
@@ -90,6 +139,7 @@ __device__ __forceinline__ float comp(const float4 &v, int c) {
 __global__ __launch_bounds__(256) void k_sr_upscale(SrArgs a) {
     __shared__ __align__(16) float act[64 * LD];
     __shared__ float skip[64 * OUT];
+    __shared__ __align__(16) float Bs[128 * BS_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int x0 = blockIdx.x * 4, y0 = blockIdx.y * 4, z0 = blockIdx.z * 4;
     // 1. gather the normalised 3x3x3 patches (x fastest, then y, z; 4 channels)
@@ -133,15 +183,15 @@ __global__ __launch_bounds__(256) void k_sr_upscale(SrArgs a) {
     float *rows = act + 16 * wave * LD;
     {
         f32x4 acc[8];
-        mlp_layer<INP, 8>(rows, a.w0, a.b0, r, q, acc);
+        mlp_layer<INP, 8>(rows, a.w0, a.b0, Bs, tid, r, q, acc);
         store_relu<8>(rows, r, q, acc);
-        mlp_layer<H, 8>(rows, a.w1, a.b1, r, q, acc);
+        mlp_layer<H, 8>(rows, a.w1, a.b1, Bs, tid, r, q, acc);
         store_relu<8>(rows, r, q, acc);
-        mlp_layer<H, 8>(rows, a.w2, a.b2, r, q, acc);
+        mlp_layer<H, 8>(rows, a.w2, a.b2, Bs, tid, r, q, acc);
         store_relu<8>(rows, r, q, acc);
     }
     f32x4 out[2];
-    mlp_layer<H, 2>(rows, a.w3, a.b3, r, q, out);
+    mlp_layer<H, 2>(rows, a.w3, a.b3, Bs, tid, r, q, out);
     // 4. skip + correction, de-normalised, to the fine cells
     const int ch = r & 3;
     const float sd = comp(a.stdv, ch), mu = comp(a.mean, ch);
