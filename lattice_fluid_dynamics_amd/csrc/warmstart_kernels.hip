@@ -177,10 +177,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
             *(uint4 *)(halo + hp * FB_LD + 8 * cg) = v;
         }
         __syncthreads();
-        // one tap: two A reads from the halo, two B reads of weight rows, four MFMAs
-        auto tap = [&](int t) {
+        // per tap: two A reads from the halo, two B reads of weight rows, four
+        // MFMAs; the operands of tap t + 1 are loaded before tap t's MFMAs
+        auto load = [&](int t, Pack8 (&a)[2], Pack8 (&b)[2]) {
             const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets (tap + 1)
-            Pack8 a[2], b[2];
 #pragma unroll
             for (int mb = 0; mb < 2; mb++) {
                 const int hp = ((wave + dz) * 6 + 2 * mb + (r >> 3) + dy) * 10 + (r & 7) + dx;
@@ -189,18 +189,26 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
                 b[nb].u = *(const uint4 *)(Wt + ((long long)(co0 + 16 * nb + r) * 27 + t) * cin + c0 + 8 * kq);
+        };
+        auto mma = [&](const Pack8 (&a)[2], const Pack8 (&b)[2]) {
 #pragma unroll
             for (int mb = 0; mb < 2; mb++)
 #pragma unroll
                 for (int nb = 0; nb < 2; nb++)
                     acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb].v, b[nb].v, acc[mb][nb], 0, 0, 0);
         };
-        if (SPLIT) {
-            for (int t = blockIdx.z * 27 / gridDim.z; t < (int)((blockIdx.z + 1) * 27 / gridDim.z); t++) tap(t);
-        } else {
-#pragma unroll 3
-            for (int t = 0; t < 27; t++) tap(t);
+        const int tbeg = SPLIT ? blockIdx.z * 27 / gridDim.z : 0;
+        const int tend = SPLIT ? (blockIdx.z + 1) * 27 / gridDim.z : 27;
+        Pack8 a0[2], b0[2], a1[2], b1[2];
+        load(tbeg, a0, b0);
+        int t = tbeg;
+        for (; t + 2 <= tend; t += 2) {
+            load(t + 1, a1, b1);
+            mma(a0, b0);
+            if (t + 2 < tend) load(t + 2, a0, b0);
+            mma(a1, b1);
         }
+        if (t < tend) mma(a0, b0);
         __syncthreads();
     }
     // D[row = position 4 kq + j][col = channel r]
