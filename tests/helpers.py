@@ -89,3 +89,37 @@ def compare(g, og, f64=False):
         "u_max": float(np.max(np.abs(vel[:, :3].astype(np.float64) - og.vel[:, :3]))),
     }
     return out
+
+
+def taylor_green(nx, ny, nz, U, mode=1):
+    """Taylor-Green vortex in the periodic y-z plane, uniform in x: u_y = U sin(k y)
+    cos(k z), u_z = -U cos(k y) sin(k z) at cell centres, k = 2 pi mode / ny (ny = nz),
+    with the matching pressure rho = 1 - (3/4) U^2 (cos 2ky + cos 2kz).  Returns the
+    (nz, ny, nx, 4) field (u, rho) and k.  Its velocity amplitude decays as
+    exp(-2 nu k^2 t) under the Navier-Stokes equations."""
+    assert ny == nz
+    k = 2 * np.pi * mode / ny
+    z, y = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, indexing="ij")
+    out = np.zeros((nz, ny, nx, 4))
+    out[..., 1] = (U * np.sin(k * y) * np.cos(k * z))[..., None]
+    out[..., 2] = (-U * np.cos(k * y) * np.sin(k * z))[..., None]
+    out[..., 3] = (1 - 0.75 * U * U * (np.cos(2 * k * y) + np.cos(2 * k * z)))[..., None]
+    return out, k
+
+
+def equilibrium(field):
+    """f_i = w_i rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2) of a (..., 4) field, (..., 19)."""
+    u, rho = field[..., :3], field[..., 3]
+    eu = u @ O.D3Q19_E.T.astype(np.float64)
+    u2 = np.sum(u * u, axis=-1)[..., None]
+    return O.D3Q19_W * rho[..., None] * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * u2)
+
+
+def tg_amplitude(vel, shape, k, x0, x1):
+    """Least-squares amplitude of the Taylor-Green mode in the cells x0 <= x < x1
+    of a (ncell, 4) velocity field."""
+    nx, ny, nz = shape
+    ref, _ = taylor_green(nx, ny, nz, 1.0, mode=round(k * ny / (2 * np.pi)))
+    v = np.asarray(vel, np.float64).reshape(nz, ny, nx, 4)[:, :, x0:x1, 1:3]
+    r = ref[:, :, x0:x1, 1:3]
+    return float(np.sum(v * r) / np.sum(r * r))

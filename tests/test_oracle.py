@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from tests import helpers as H
 
 E = O.D3Q19_E
 OPP = O.D3Q19_OPP
@@ -375,6 +376,45 @@ def test_channel_wall_position_follows_q(qlow, qhigh):
     assert abs(roots[1] - (18 + qhigh)) < 0.12, roots
     resid = u - np.polyval(c, z)
     assert np.max(np.abs(resid)) < 0.02 * np.max(u)
+
+
+def taylor_green_rate(op, tau, steps=(20, 120), nx=40, n=32, U=0.02):
+    """Decay rate per step of the Taylor-Green mode (u_y, u_z in the periodic y-z
+    plane, uniform in x) measured in the slab 12 <= x < 28, away from the inlet
+    and outlet faces, between two sample times."""
+    field, k = H.taylor_green(nx, n, n, U)
+    g = O.OracleGrid(nx, n, n, 0.05, precision=64)
+    g.tau = tau
+    g.periodicYZ = 1
+    g.useMRT = op == "mrt"
+    g.useRegularized = op == "reg"
+    g.useSmagorinsky = op == "smag"
+    g.f[:] = H.equilibrium(field).reshape(-1, 19)
+    g.step(0, 0, 0, n=steps[0])
+    a1 = H.tg_amplitude(g.vel, (nx, n, n), k, 12, 28)
+    g.step(0, 0, 0, n=steps[1] - steps[0])
+    a2 = H.tg_amplitude(g.vel, (nx, n, n), k, 12, 28)
+    return np.log(a1 / a2) / (steps[1] - steps[0]), k
+
+
+@pytest.mark.parametrize("op", ["bgk", "mrt", "reg"])
+@pytest.mark.parametrize("tau", [0.55, 0.8])
+def test_taylor_green_decays_at_the_lattice_viscosity(op, tau):
+    """The vortex's amplitude decays as exp(-2 nu k^2 t) with nu = (tau - 1/2)/3
+    for all three collision operators (32 cells per wavelength: the lattice
+    leaves under 1 % at these tau).  A relaxation rate on the wrong moments
+    or the wrong viscosity relation moves the rate by far more."""
+    rate, k = taylor_green_rate(op, tau)
+    nu = (tau - 0.5) / 3
+    assert abs(rate / (2 * nu * k * k) - 1) < 0.01, rate / (2 * nu * k * k)
+
+
+def test_smagorinsky_only_adds_viscosity_to_the_vortex():
+    """Smagorinsky raises tau_eff above tau wherever the strain is non-zero, so the
+    vortex decays faster than plain BGK, but by little on a smooth, slow flow."""
+    lam, _ = taylor_green_rate("bgk", 0.55)
+    smag, _ = taylor_green_rate("smag", 0.55)
+    assert lam < smag < 1.02 * lam
 
 
 # ------------------------------------------------------------ force
