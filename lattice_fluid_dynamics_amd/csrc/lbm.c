@@ -829,6 +829,20 @@ int LBM_SetBlockSize(LBMGrid *g, int bs) {
     return 1;
 }
 
+int LBM_SetStreamingAccess(LBMGrid *g, int on) {
+    if (!g || !impl_of(g)->dev) return 0;
+    lbmdev_set_streaming(impl_of(g)->dev, on);
+    return 1;
+}
+
+int LBM_GetKernelConfig(const LBMGrid *g, int *blockSize, int *streaming) {
+    if (!g || !impl_of((LBMGrid *)g)->dev) return 0;
+    const LbmDev *d = impl_of((LBMGrid *)g)->dev;
+    if (blockSize) *blockSize = lbmdev_block_size(d);
+    if (streaming) *streaming = lbmdev_streaming(d);
+    return 1;
+}
+
 /* Attainable HBM bandwidth: a plain device copy of `bytes` bytes, read +
  * write bytes over the mean launch time. */
 float LBM_MeasureCopyBandwidth(unsigned long long bytes, int iters) {

@@ -84,6 +84,8 @@ int lbmdev_set_block_size(LbmDev *d, int bs);
  * the populations and velocity after each trial; sets and returns the best. */
 int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, float *ms_per_step);
 int lbmdev_block_size(const LbmDev *d);
+void lbmdev_set_streaming(LbmDev *d, int on);
+int lbmdev_streaming(const LbmDev *d);
 int lbmdev_export_populations_f64(LbmDev *d, int periodicYZ, double *f_post_stream, double *f_post_collision);
 void *lbmdev_stream(LbmDev *d);
 

@@ -170,33 +170,44 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base) {
     // gfx9-family resource: stride 0, no range limit, raw 32-bit data format
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, -1, 0x00020000);
 }
+// CP is the instruction's cache-policy field: 0 = default, CP_NT = the
+// non-temporal bit (lines are streamed through L2 and the memory-side cache
+// instead of being kept).  With both the 19 loads and the 19 stores of a cell
+// non-temporal, a lattice far larger than the caches whose stores cover whole
+// lines moves ~6 % more bytes per second; lattices that partly fit the caches,
+// or whose stores are partial lines, lose 7-11 %.  The autotuner measures both.
+constexpr int CP_NT = 2;
+template <int CP>
 __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, float) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, CP));
 }
+template <int CP>
 __device__ __forceinline__ uint16_t buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint16_t) {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, CP);
 }
+template <int CP>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, CP);
 }
+template <int CP>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, uint16_t v) {
-    __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, CP);
 }
 
-template <typename S>
+template <typename S, int CP = 0>
 struct SlotIO {
     S *pop;
     long long stride;
     __device__ __forceinline__ SlotIO(S *p, const Geo &g) : pop(p), stride(g.stride) {}
     __device__ __forceinline__ S ld(int i, uint32_t cell) const {
-        return buf_ld(plane_rsrc(pop + (long long)i * stride), cell * (uint32_t)sizeof(S), S());
+        return buf_ld<CP>(plane_rsrc(pop + (long long)i * stride), cell * (uint32_t)sizeof(S), S());
     }
     __device__ __forceinline__ void st(int i, uint32_t cell, S v) const {
-        buf_st(plane_rsrc(pop + (long long)i * stride), cell * (uint32_t)sizeof(S), v);
+        buf_st<CP>(plane_rsrc(pop + (long long)i * stride), cell * (uint32_t)sizeof(S), v);
     }
 };
-template <>
-struct SlotIO<double> {
+template <int CP>
+struct SlotIO<double, CP> {
     double *pop;
     long long stride;
     __device__ __forceinline__ SlotIO(double *p, const Geo &g) : pop(p), stride(g.stride) {}
@@ -513,7 +524,7 @@ struct StepArgs {
 constexpr int XPLUS[5] = {1, 7, 9, 11, 13};
 constexpr int XMINUS[5] = {2, 8, 10, 12, 14};
 
-template <int FMT, int OP, int PHASE, int ROW = 0>
+template <int FMT, int OP, int PHASE, int ROW = 0, int NT = 0>
 __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell) {
     typedef Pop<FMT> P;
     typedef typename P::T T;
@@ -528,7 +539,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
 
     const uint8_t m = a.mask[cell];
     const bool ovr = a.use_ovr && (m & LBMDEV_MASK_OVR);
-    const SlotIO<typename P::S> io(a.pop, g);
+    const SlotIO<typename P::S, NT ? CP_NT : 0> io(a.pop, g);
     T f[19];
     if (ovr) {
         const int k = a.ovr_index[cell];
@@ -614,11 +625,12 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
 // The fused kernel over one lattice, BS lanes per workgroup.
 // ROW = 1 (phase B only): row-aligned stores; the launch guarantees that
 // every workgroup holds whole rows and no lane is past the lattice.
-template <int FMT, int OP, int PHASE, int BS, int ROW = 0>
+// NT = 1: non-temporal population loads and stores.
+template <int FMT, int OP, int PHASE, int BS, int ROW = 0, int NT = 0>
 __global__ __launch_bounds__(BS) void k_step(StepArgs<FMT> a) {
     const int cell = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (!ROW && cell >= a.g.ncell) return;
-    step_cell<FMT, OP, PHASE, ROW>(a, cell);
+    step_cell<FMT, OP, PHASE, ROW, NT>(a, cell);
 }
 
 // ---------------------------------------------------------- batched tunnels
@@ -1131,6 +1143,7 @@ struct LbmDev {
     float *feq_d;
     int nparts;  // partial slots allocated
     int bs;      // workgroup size of the step kernel
+    int nt;      // non-temporal population loads/stores in the step kernel
 };
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
@@ -1365,30 +1378,30 @@ static void launch_prepass(LbmDev *d, int periodic) {
         hipLaunchKernelGGL((k_prepass<FMT, ST_B>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
 }
 
-template <int FMT, int OP, int BS>
+template <int FMT, int OP, int BS, int NT>
 static void launch_step_phase(int phase, int ncell, hipStream_t st, const StepArgs<FMT> &a) {
     const int nb = (ncell + BS - 1) / BS;
     // whole rows per workgroup, rows of whole wavefronts, no partial workgroup
     const bool row = FMT != LBMDEV_FP64 && a.g.nx % 64 == 0 && BS % a.g.nx == 0 && ncell % BS == 0;
     if (phase == 0)
-        hipLaunchKernelGGL((k_step<FMT, OP, 0, BS>), dim3(nb), dim3(BS), 0, st, a);
+        hipLaunchKernelGGL((k_step<FMT, OP, 0, BS, 0, NT>), dim3(nb), dim3(BS), 0, st, a);
     else if (row)
-        hipLaunchKernelGGL((k_step<FMT, OP, 1, BS, 1>), dim3(nb), dim3(BS), 0, st, a);
+        hipLaunchKernelGGL((k_step<FMT, OP, 1, BS, 1, NT>), dim3(nb), dim3(BS), 0, st, a);
     else
-        hipLaunchKernelGGL((k_step<FMT, OP, 1, BS>), dim3(nb), dim3(BS), 0, st, a);
+        hipLaunchKernelGGL((k_step<FMT, OP, 1, BS, 0, NT>), dim3(nb), dim3(BS), 0, st, a);
 }
 
 // Workgroup sizes the autotuner chooses from.
 static const int kBlockSizes[3] = {128, 256, 512};
 
-template <int FMT, int OP>
+template <int FMT, int OP, int NT>
 static void launch_step_bs(int bs, int phase, int ncell, hipStream_t st, const StepArgs<FMT> &a) {
     if (bs == 128)
-        launch_step_phase<FMT, OP, 128>(phase, ncell, st, a);
+        launch_step_phase<FMT, OP, 128, NT>(phase, ncell, st, a);
     else if (bs == 512)
-        launch_step_phase<FMT, OP, 512>(phase, ncell, st, a);
+        launch_step_phase<FMT, OP, 512, NT>(phase, ncell, st, a);
     else
-        launch_step_phase<FMT, OP, 256>(phase, ncell, st, a);
+        launch_step_phase<FMT, OP, 256, NT>(phase, ncell, st, a);
 }
 
 template <int FMT, int OP>
@@ -1409,7 +1422,11 @@ static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ov
     a.cp.iny = p->inletY;
     a.cp.inz = p->inletZ;
     a.cp.smag = p->useSmagorinsky == 1;
-    launch_step_bs<FMT, OP>(d->bs, phase, d->ncell, d->stream, a);
+    // FP64 storage uses global addressing and has no streaming variant
+    if (FMT != LBMDEV_FP64 && d->nt)
+        launch_step_bs<FMT, OP, FMT != LBMDEV_FP64>(d->bs, phase, d->ncell, d->stream, a);
+    else
+        launch_step_bs<FMT, OP, 0>(d->bs, phase, d->ncell, d->stream, a);
 }
 
 template <int FMT>
@@ -2337,6 +2354,10 @@ extern "C" int lbmdev_set_block_size(LbmDev *d, int bs) {
 
 extern "C" int lbmdev_block_size(const LbmDev *d) { return d->bs; }
 
+extern "C" void lbmdev_set_streaming(LbmDev *d, int on) { d->nt = d->fmt != LBMDEV_FP64 && on; }
+
+extern "C" int lbmdev_streaming(const LbmDev *d) { return d->nt; }
+
 // ---------------------------------------------------------------- autotuner
 // Candidates are timed on the real lattice; the stored state (populations,
 // velocity, layout) is copied aside first and put back after every trial.
@@ -2373,19 +2394,23 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
     Snapshot s;
     const size_t pb = (size_t)19 * d->stride * d->elem, vb = (size_t)d->ncell * 16;
     if (!snapshot_take(s, d->pop, pb, d->vel, vb, d->stream)) return 0;
-    const int state0 = d->state, bs0 = d->bs;
-    int best = 0;
+    const int state0 = d->state, bs0 = d->bs, nt0 = d->nt;
+    int best = 0, bestNt = 0;
     float bestMs = 0;
     int ok = 1;
-    for (int c = 0; c < 3 && ok; c++) {
-        d->bs = kBlockSizes[c];
+    // every workgroup size with default, then non-temporal population access
+    const int ncand = d->fmt == LBMDEV_FP64 ? 3 : 6;
+    for (int c = 0; c < ncand && ok; c++) {
+        d->bs = kBlockSizes[c % 3];
+        d->nt = c / 3;
         float tot = 0;
         ok = dispatch_steps(d, p, 2, NULL) &&  // warm-up
              lbmdev_profile_steps(d, p, steps, &tot, NULL, NULL);
         const float ms = tot / steps;
-        if (ms_per_step) ms_per_step[c] = ms;
+        if (ms_per_step && (c < 3 || ms < ms_per_step[c % 3])) ms_per_step[c % 3] = ms;
         if (ok && (best == 0 || ms < bestMs)) {
-            best = kBlockSizes[c];
+            best = d->bs;
+            bestNt = d->nt;
             bestMs = ms;
         }
         ok = ok && snapshot_put(s, d->pop, d->vel, d->stream);
@@ -2394,6 +2419,7 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
     hipFree(s.pop);
     hipFree(s.vel);
     d->bs = ok ? best : bs0;
+    d->nt = ok ? bestNt : nt0;
     return ok ? best : 0;
 }
 

@@ -98,6 +98,8 @@ SIGNATURES = {
     "LBM_WriteVTI": (c_int, [GridP, ctypes.c_char_p, c_int]),
     "LBM_Autotune": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p]),
     "LBM_SetBlockSize": (c_int, [GridP, c_int]),
+    "LBM_SetStreamingAccess": (c_int, [GridP, c_int]),
+    "LBM_GetKernelConfig": (c_int, [GridP, c_int_p, c_int_p]),
     "LBM_MeasureCopyBandwidth": (c_float, [ctypes.c_ulonglong, c_int]),
     "LBM_BatchCreate": (c_void_p, [c_int, c_int, c_int, c_int, c_float, c_int]),
     "LBM_BatchFree": (None, [c_void_p]),
@@ -429,8 +431,22 @@ def LBM_SetBlockSize(g: Grid, bs: int) -> None:
         raise ValueError(last_error())
 
 
+def LBM_SetStreamingAccess(g: Grid, on: bool) -> None:
+    if not load_library().LBM_SetStreamingAccess(g.ptr, int(on)):
+        raise ValueError(last_error())
+
+
+def LBM_GetKernelConfig(g: Grid):
+    """(workgroup size, streaming flag) of the step kernel."""
+    bs, nt = c_int(), c_int()
+    if not load_library().LBM_GetKernelConfig(g.ptr, ctypes.byref(bs), ctypes.byref(nt)):
+        raise RuntimeError(last_error())
+    return bs.value, bool(nt.value)
+
+
 def LBM_Autotune(g: Grid, steps: int, ux: float, uy: float = 0.0, uz: float = 0.0):
-    """Returns (chosen workgroup size, ms per step for 128/256/512 lanes)."""
+    """Returns (chosen workgroup size, best ms per step for 128/256/512 lanes);
+    the chosen streaming flag is read back with LBM_GetKernelConfig."""
     ms = np.zeros(3, np.float32)
     bs = load_library().LBM_Autotune(g.ptr, steps, ux, uy, uz, _fp(ms))
     if not bs:
