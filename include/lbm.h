@@ -225,12 +225,12 @@ int LBM_WriteVTI(LBMGrid *grid, const char *dir, int step);
 /* ---- workgroup-size autotuner (no reference counterpart) ------------------ */
 
 /* Times `steps` steps of the grid for each candidate configuration of the
- * step kernel: workgroup size 128, 256 or 512 lanes, each with default and
- * with non-temporal (streaming) population loads and stores (FP64 storage:
- * default only).  Uses the given inlet velocity, restores the simulation
- * state after each trial, keeps the fastest configuration and returns its
- * workgroup size (0 on failure).  msPerStep (3 floats, may be NULL) receives
- * the best time per workgroup size. */
+ * step kernel: workgroup size 128, 256 or 512 lanes, each with streaming
+ * mode 0, 1 and 3 (see LBM_SetStreamingAccess; FP64 storage: mode 0 only).
+ * Uses the given inlet velocity, restores the simulation state after each
+ * trial, keeps the fastest configuration and returns its workgroup size
+ * (0 on failure).  msPerStep (3 floats, may be NULL) receives the best time
+ * per workgroup size. */
 int LBM_Autotune(LBMGrid *grid, int steps, float inletVelX, float inletVelY, float inletVelZ,
                  float *msPerStep);
 
@@ -238,14 +238,16 @@ int LBM_Autotune(LBMGrid *grid, int steps, float inletVelX, float inletVelY, flo
  * tuning it; the results do not depend on it.  1 on success. */
 int LBM_SetBlockSize(LBMGrid *grid, int blockSize);
 
-/* Selects non-temporal (on = 1) or default (on = 0) cache policy for the step
- * kernel's population loads and stores instead of tuning it; FP64 storage
- * keeps the default.  The results do not depend on it.  1 on success. */
-int LBM_SetStreamingAccess(LBMGrid *grid, int on);
+/* Selects the cache policy of the step kernel's population loads and stores
+ * instead of tuning it: mode bit 0 makes them non-temporal in phase A (own
+ * slots), bit 1 in phase B (neighbour slots); 0 is the default policy.  FP64
+ * storage keeps the default.  The results do not depend on it.  1 on success,
+ * 0 for a mode outside 0-3. */
+int LBM_SetStreamingAccess(LBMGrid *grid, int mode);
 
-/* The step kernel's current workgroup size and streaming flag (either
+/* The step kernel's current workgroup size and streaming mode (either
  * pointer may be NULL).  1 on success. */
-int LBM_GetKernelConfig(const LBMGrid *grid, int *blockSize, int *streaming);
+int LBM_GetKernelConfig(const LBMGrid *grid, int *blockSize, int *streamingMode);
 
 /* Attainable HBM bandwidth in GB/s: read + write bytes of a plain device copy
  * of `bytes` bytes over its mean launch time (iters launches). */

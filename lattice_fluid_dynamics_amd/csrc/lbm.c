@@ -829,9 +829,12 @@ int LBM_SetBlockSize(LBMGrid *g, int bs) {
     return 1;
 }
 
-int LBM_SetStreamingAccess(LBMGrid *g, int on) {
+int LBM_SetStreamingAccess(LBMGrid *g, int mode) {
     if (!g || !impl_of(g)->dev) return 0;
-    lbmdev_set_streaming(impl_of(g)->dev, on);
+    if (!lbmdev_set_streaming(impl_of(g)->dev, mode)) {
+        note_error("LBM_SetStreamingAccess: the mode must be 0 to 3");
+        return 0;
+    }
     return 1;
 }
 

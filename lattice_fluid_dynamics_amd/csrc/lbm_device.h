@@ -19,6 +19,8 @@ extern "C" {
 
 /* population storage formats */
 enum { LBMDEV_FP32 = 0, LBMDEV_FP16S = 1, LBMDEV_BF16S = 2, LBMDEV_FP64 = 3 };
+/* streaming (non-temporal) population access of the step kernel, per phase */
+enum { LBMDEV_NT_A = 1, LBMDEV_NT_B = 2 };
 
 /* mask byte bits */
 enum { LBMDEV_MASK_BODY = 1, LBMDEV_MASK_GROUND = 2, LBMDEV_MASK_OVR = 4 };
@@ -84,7 +86,7 @@ int lbmdev_set_block_size(LbmDev *d, int bs);
  * the populations and velocity after each trial; sets and returns the best. */
 int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, float *ms_per_step);
 int lbmdev_block_size(const LbmDev *d);
-void lbmdev_set_streaming(LbmDev *d, int on);
+int lbmdev_set_streaming(LbmDev *d, int mode);
 int lbmdev_streaming(const LbmDev *d);
 int lbmdev_export_populations_f64(LbmDev *d, int periodicYZ, double *f_post_stream, double *f_post_collision);
 void *lbmdev_stream(LbmDev *d);

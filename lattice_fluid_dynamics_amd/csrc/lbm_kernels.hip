@@ -172,10 +172,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base) {
 }
 // CP is the instruction's cache-policy field: 0 = default, CP_NT = the
 // non-temporal bit (lines are streamed through L2 and the memory-side cache
-// instead of being kept).  With both the 19 loads and the 19 stores of a cell
-// non-temporal, a lattice far larger than the caches whose stores cover whole
-// lines moves ~6 % more bytes per second; lattices that partly fit the caches,
-// or whose stores are partial lines, lose 7-11 %.  The autotuner measures both.
+// instead of being kept).  It pays only when a cell's 19 loads and 19 stores
+// both carry it.  Phase A (own slots, whole-line stores) gains 5-8 % on
+// lattices far larger than the caches; phase B gains only with row-aligned
+// stores and loses up to 10 % with per-cell (partial-line) stores; lattices
+// that partly fit the caches lose.  The choice per phase is a launch
+// parameter (LBMDEV_NT_A / LBMDEV_NT_B) that the autotuner measures.
 constexpr int CP_NT = 2;
 template <int CP>
 __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, float) {
@@ -1143,7 +1145,7 @@ struct LbmDev {
     float *feq_d;
     int nparts;  // partial slots allocated
     int bs;      // workgroup size of the step kernel
-    int nt;      // non-temporal population loads/stores in the step kernel
+    int nt;      // LBMDEV_NT_A / LBMDEV_NT_B: non-temporal population access per phase
 };
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
@@ -1404,6 +1406,9 @@ static void launch_step_bs(int bs, int phase, int ncell, hipStream_t st, const S
         launch_step_phase<FMT, OP, 256, NT>(phase, ncell, st, a);
 }
 
+// Streaming modes the autotuner chooses from: none, phase A only, both.
+static const int kStreamModes[3] = {0, LBMDEV_NT_A, LBMDEV_NT_A | LBMDEV_NT_B};
+
 template <int FMT, int OP>
 static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ovr, int write_vel) {
     StepArgs<FMT> a;
@@ -1423,7 +1428,7 @@ static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ov
     a.cp.inz = p->inletZ;
     a.cp.smag = p->useSmagorinsky == 1;
     // FP64 storage uses global addressing and has no streaming variant
-    if (FMT != LBMDEV_FP64 && d->nt)
+    if (FMT != LBMDEV_FP64 && (d->nt & (phase == 0 ? LBMDEV_NT_A : LBMDEV_NT_B)))
         launch_step_bs<FMT, OP, FMT != LBMDEV_FP64>(d->bs, phase, d->ncell, d->stream, a);
     else
         launch_step_bs<FMT, OP, 0>(d->bs, phase, d->ncell, d->stream, a);
@@ -2354,7 +2359,11 @@ extern "C" int lbmdev_set_block_size(LbmDev *d, int bs) {
 
 extern "C" int lbmdev_block_size(const LbmDev *d) { return d->bs; }
 
-extern "C" void lbmdev_set_streaming(LbmDev *d, int on) { d->nt = d->fmt != LBMDEV_FP64 && on; }
+extern "C" int lbmdev_set_streaming(LbmDev *d, int mode) {
+    if (mode < 0 || mode > (LBMDEV_NT_A | LBMDEV_NT_B)) return 0;
+    d->nt = d->fmt == LBMDEV_FP64 ? 0 : mode;
+    return 1;
+}
 
 extern "C" int lbmdev_streaming(const LbmDev *d) { return d->nt; }
 
@@ -2398,11 +2407,11 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
     int best = 0, bestNt = 0;
     float bestMs = 0;
     int ok = 1;
-    // every workgroup size with default, then non-temporal population access
-    const int ncand = d->fmt == LBMDEV_FP64 ? 3 : 6;
+    // every workgroup size with each streaming mode (FP64: default access only)
+    const int ncand = d->fmt == LBMDEV_FP64 ? 3 : 9;
     for (int c = 0; c < ncand && ok; c++) {
         d->bs = kBlockSizes[c % 3];
-        d->nt = c / 3;
+        d->nt = kStreamModes[c / 3];
         float tot = 0;
         ok = dispatch_steps(d, p, 2, NULL) &&  // warm-up
              lbmdev_profile_steps(d, p, steps, &tot, NULL, NULL);

@@ -431,22 +431,26 @@ def LBM_SetBlockSize(g: Grid, bs: int) -> None:
         raise ValueError(last_error())
 
 
-def LBM_SetStreamingAccess(g: Grid, on: bool) -> None:
-    if not load_library().LBM_SetStreamingAccess(g.ptr, int(on)):
+STREAM_PHASE_A, STREAM_PHASE_B = 1, 2
+
+
+def LBM_SetStreamingAccess(g: Grid, mode: int) -> None:
+    """mode: STREAM_PHASE_A | STREAM_PHASE_B bits, 0 = default cache policy."""
+    if not load_library().LBM_SetStreamingAccess(g.ptr, int(mode)):
         raise ValueError(last_error())
 
 
 def LBM_GetKernelConfig(g: Grid):
-    """(workgroup size, streaming flag) of the step kernel."""
+    """(workgroup size, streaming mode) of the step kernel."""
     bs, nt = c_int(), c_int()
     if not load_library().LBM_GetKernelConfig(g.ptr, ctypes.byref(bs), ctypes.byref(nt)):
         raise RuntimeError(last_error())
-    return bs.value, bool(nt.value)
+    return bs.value, nt.value
 
 
 def LBM_Autotune(g: Grid, steps: int, ux: float, uy: float = 0.0, uz: float = 0.0):
     """Returns (chosen workgroup size, best ms per step for 128/256/512 lanes);
-    the chosen streaming flag is read back with LBM_GetKernelConfig."""
+    the chosen streaming mode is read back with LBM_GetKernelConfig."""
     ms = np.zeros(3, np.float32)
     bs = load_library().LBM_Autotune(g.ptr, steps, ux, uy, uz, _fp(ms))
     if not bs:

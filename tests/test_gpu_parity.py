@@ -444,10 +444,11 @@ def test_row_aligned_phase_b_is_bitwise_identical(gpu, op):
 
 @pytest.mark.parametrize("storage", [0, 1, 3])
 def test_streaming_access_and_autotune_leave_the_results_unchanged(gpu, storage):
-    """Non-temporal population loads/stores change only the cache policy: both
-    settings at every workgroup size (row-aligned and per-cell phase-B stores)
-    give bit-identical fields.  The autotuner, which times all of them, restores
-    the state it started from, so a tuned run also matches bit for bit."""
+    """Non-temporal population loads/stores change only the cache policy: every
+    streaming mode at every workgroup size (row-aligned and per-cell phase-B
+    stores) gives bit-identical fields.  The autotuner, which times them,
+    restores the state it started from, so a tuned run also matches bit for
+    bit.  Modes outside 0-3 are rejected."""
     def run(bs, nt, tune=False):
         g = gpu.LBM_Create(512, 16, 16, 0.02)
         g.tau = 0.56
@@ -462,16 +463,18 @@ def test_streaming_access_and_autotune_leave_the_results_unchanged(gpu, storage)
             best, ms = gpu.LBM_Autotune(g, 4, 0.05)
             cfg = gpu.LBM_GetKernelConfig(g)
             assert cfg[0] == best and np.all(ms > 0)
-            assert not cfg[1] or storage != 3  # FP64 storage has no streaming variant
+            assert cfg[1] in (0, 1, 3) and (cfg[1] == 0 or storage != 3)  # FP64: default only
         else:
             gpu.LBM_SetBlockSize(g, bs)
             gpu.LBM_SetStreamingAccess(g, nt)
-            assert gpu.LBM_GetKernelConfig(g) == (bs, nt and storage != 3)
+            assert gpu.LBM_GetKernelConfig(g) == (bs, nt if storage != 3 else 0)
+            with pytest.raises(ValueError):
+                gpu.LBM_SetStreamingAccess(g, 4)
         gpu.LBM_StepN(g, 6, 0.05, 0, 0)
         return fields(g, f64=storage == 3)
-    ref = run(512, False)
-    for bs, nt in ((512, True), (256, True), (128, True), (256, False)):
+    ref = run(512, 0)
+    for bs, nt in ((512, 3), (512, 1), (512, 2), (256, 3), (128, 1), (256, 0)):
         got = run(bs, nt)
         assert all(np.array_equal(a, b) for a, b in zip(ref, got)), (bs, nt)
-    got = run(0, False, tune=True)
+    got = run(0, 0, tune=True)
     assert all(np.array_equal(a, b) for a, b in zip(ref, got)), "autotuned"
