@@ -282,6 +282,11 @@ void LBM_BatchStep(LBMBatch *batch, int n);
 int LBM_BatchProfile(LBMBatch *batch, int n, int useGraph, float *ms);
 int LBM_BatchAutotune(LBMBatch *batch, int steps, float *msPerStep);
 int LBM_BatchSetBlockSize(LBMBatch *batch, int blockSize);
+/* As LBM_SetStreamingAccess and LBM_GetKernelConfig, for the batch step
+ * kernel (LBM_BatchAutotune tunes the streaming mode too).  Available after
+ * LBM_BatchInitialize. */
+int LBM_BatchSetStreamingAccess(LBMBatch *batch, int mode);
+int LBM_BatchGetKernelConfig(const LBMBatch *batch, int *blockSize, int *streamingMode);
 int LBM_BatchTunnelCount(const LBMBatch *batch);
 long long LBM_BatchTotalCells(const LBMBatch *batch);
 /* out: 6 doubles per tunnel (F xyz, pressure part xyz, FP64 sums); fixedOut

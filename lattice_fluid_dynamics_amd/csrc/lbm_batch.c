@@ -236,6 +236,17 @@ int LBM_BatchAutotune(LBMBatch *b, int steps, float *msPerStep) {
 
 int LBM_BatchSetBlockSize(LBMBatch *b, int bs) { return b && b->dev ? lbmbatch_set_block_size(b->dev, bs) : 0; }
 
+int LBM_BatchSetStreamingAccess(LBMBatch *b, int mode) {
+    return b && b->dev ? lbmbatch_set_streaming(b->dev, mode) : 0;
+}
+
+int LBM_BatchGetKernelConfig(const LBMBatch *b, int *blockSize, int *streamingMode) {
+    if (!b || !b->dev) return 0;
+    if (blockSize) *blockSize = lbmbatch_block_size(b->dev);
+    if (streamingMode) *streamingMode = lbmbatch_streaming(b->dev);
+    return 1;
+}
+
 int LBM_BatchTunnelCount(const LBMBatch *b) { return b ? b->nt : 0; }
 
 long long LBM_BatchTotalCells(const LBMBatch *b) {

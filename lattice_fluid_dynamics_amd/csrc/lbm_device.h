@@ -113,6 +113,8 @@ int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int us
 int lbmbatch_set_block_size(LbmBatchDev *b, int bs);
 int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int steps, float *ms_per_step);
 int lbmbatch_block_size(const LbmBatchDev *b);
+int lbmbatch_set_streaming(LbmBatchDev *b, int mode);
+int lbmbatch_streaming(const LbmBatchDev *b);
 int lbmbatch_force(LbmBatchDev *b, int periodic, double *exact, long long *fixed);
 int lbmbatch_export(LbmBatchDev *b, int t, int periodic, double *f_post_stream, double *f_post_collision);
 int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host);

@@ -665,7 +665,8 @@ struct BatchArgs {
 };
 
 // ROW = 1 (phase B only): every tunnel's rows fill whole workgroups.
-template <int FMT, int OP, int PHASE, int BS, int ROW = 0>
+// NT = 1: non-temporal population loads and stores.
+template <int FMT, int OP, int PHASE, int BS, int ROW = 0, int NT = 0>
 __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const TunnelDesc d = b.desc[b.block_tunnel[blk]];
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     a.cp.iny = d.iny;
     a.cp.inz = d.inz;
     a.cp.smag = b.smag;
-    step_cell<FMT, OP, PHASE, ROW>(a, cell);
+    step_cell<FMT, OP, PHASE, ROW, NT>(a, cell);
 }
 
 template <int FMT>
@@ -1797,6 +1798,7 @@ struct LbmBatchDev {
     void *ovr_vals;
     int32_t *block_tunnel;
     int nblocks, bs;
+    int smode;  // LBMDEV_NT_A / LBMDEV_NT_B streaming mode of the step kernel
     int state;
     hipStream_t stream;
     double *pd;
@@ -1807,7 +1809,7 @@ struct LbmBatchDev {
     hipGraphExec_t graph[2];
     int graph_ok;
     LbmBatchParams graph_params;
-    int graph_bs;
+    int graph_bs, graph_smode;
 };
 
 extern "C" void lbmbatch_destroy(LbmBatchDev *b);
@@ -2064,7 +2066,7 @@ extern "C" int lbmbatch_init_tunnel_field(LbmBatchDev *b, int t, const float *co
     return 1;
 }
 
-template <int FMT, int OP, int BS>
+template <int FMT, int OP, int BS, int NT>
 static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase, int use_ovr, int wv) {
     BatchArgs<FMT> a;
     a.pop = (typename Pop<FMT>::S *)b->pop;
@@ -2086,21 +2088,30 @@ static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase
         row = g.nx % 64 == 0 && BS % g.nx == 0 && g.ncell % BS == 0;
     }
     if (phase == 0)
-        hipLaunchKernelGGL((k_step_batch<FMT, OP, 0, BS>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+        hipLaunchKernelGGL((k_step_batch<FMT, OP, 0, BS, 0, NT>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
     else if (row)
-        hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS, 1>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+        hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS, 1, NT>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
     else
-        hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+        hipLaunchKernelGGL((k_step_batch<FMT, OP, 1, BS, 0, NT>), dim3(b->nblocks), dim3(BS), 0, b->stream, a);
+}
+
+template <int FMT, int OP, int NT>
+static void batch_launch_main_nt(LbmBatchDev *b, const LbmBatchParams *p, int phase, int use_ovr, int wv) {
+    if (b->bs == 128)
+        batch_launch_main<FMT, OP, 128, NT>(b, p, phase, use_ovr, wv);
+    else if (b->bs == 512)
+        batch_launch_main<FMT, OP, 512, NT>(b, p, phase, use_ovr, wv);
+    else
+        batch_launch_main<FMT, OP, 256, NT>(b, p, phase, use_ovr, wv);
 }
 
 template <int FMT, int OP>
 static void batch_launch_main_bs(LbmBatchDev *b, const LbmBatchParams *p, int phase, int use_ovr, int wv) {
-    if (b->bs == 128)
-        batch_launch_main<FMT, OP, 128>(b, p, phase, use_ovr, wv);
-    else if (b->bs == 512)
-        batch_launch_main<FMT, OP, 512>(b, p, phase, use_ovr, wv);
+    // FP64 storage uses global addressing and has no streaming variant
+    if (FMT != LBMDEV_FP64 && (b->smode & (phase == 0 ? LBMDEV_NT_A : LBMDEV_NT_B)))
+        batch_launch_main_nt<FMT, OP, FMT != LBMDEV_FP64>(b, p, phase, use_ovr, wv);
     else
-        batch_launch_main<FMT, OP, 256>(b, p, phase, use_ovr, wv);
+        batch_launch_main_nt<FMT, OP, 0>(b, p, phase, use_ovr, wv);
 }
 
 template <int FMT>
@@ -2173,6 +2184,7 @@ static int batch_capture(LbmBatchDev *b, const LbmBatchParams *p) {
     b->graph_ok = 1;
     b->graph_params = *p;
     b->graph_bs = b->bs;
+    b->graph_smode = b->smode;
     return 1;
 }
 
@@ -2186,7 +2198,8 @@ extern "C" int lbmbatch_step(LbmBatchDev *b, const LbmBatchParams *p, int nsteps
         nsteps--;
     }
     if (use_graph && nsteps >= 2) {
-        if (!b->graph_ok || !same_params(&b->graph_params, p) || b->graph_bs != b->bs)
+        if (!b->graph_ok || !same_params(&b->graph_params, p) || b->graph_bs != b->bs ||
+            b->graph_smode != b->smode)
             if (!batch_capture(b, p)) return 0;
         const int pairs = nsteps / 2, rem = nsteps % 2;
         for (int k = 0; k < pairs; k++) {
@@ -2208,6 +2221,14 @@ extern "C" int lbmbatch_set_block_size(LbmBatchDev *b, int bs) {
 }
 
 extern "C" int lbmbatch_block_size(const LbmBatchDev *b) { return b->bs; }
+
+extern "C" int lbmbatch_set_streaming(LbmBatchDev *b, int mode) {
+    if (mode < 0 || mode > (LBMDEV_NT_A | LBMDEV_NT_B)) return 0;
+    b->smode = b->fmt == LBMDEV_FP64 ? 0 : mode;
+    return 1;
+}
+
+extern "C" int lbmbatch_streaming(const LbmBatchDev *b) { return b->smode; }
 
 extern "C" int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph, float *ms) {
     hipEvent_t t0, t1;
@@ -2437,18 +2458,22 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
     Snapshot s;
     const size_t pb = (size_t)b->total_pop * b->elem, vb = (size_t)b->total_cells * 16;
     if (!snapshot_take(s, b->pop, pb, b->vel, vb, b->stream)) return 0;
-    const int state0 = b->state, bs0 = b->bs;
-    int best = 0;
+    const int state0 = b->state, bs0 = b->bs, smode0 = b->smode;
+    int best = 0, bestMode = 0;
     float bestMs = 0;
     int ok = 1;
-    for (int c = 0; c < 3 && ok; c++) {
-        ok = lbmbatch_set_block_size(b, kBlockSizes[c]);
+    // every workgroup size with each streaming mode (FP64: default access only)
+    const int ncand = b->fmt == LBMDEV_FP64 ? 3 : 9;
+    for (int c = 0; c < ncand && ok; c++) {
+        ok = lbmbatch_set_block_size(b, kBlockSizes[c % 3]);
+        b->smode = kStreamModes[c / 3];
         float ms = 0;
         ok = ok && lbmbatch_step(b, p, 2, 1) && lbmbatch_profile(b, p, steps, 1, &ms);
         ms /= steps;
-        if (ms_per_step) ms_per_step[c] = ms;
+        if (ms_per_step && (c < 3 || ms < ms_per_step[c % 3])) ms_per_step[c % 3] = ms;
         if (ok && (best == 0 || ms < bestMs)) {
-            best = kBlockSizes[c];
+            best = kBlockSizes[c % 3];
+            bestMode = b->smode;
             bestMs = ms;
         }
         ok = ok && snapshot_put(s, b->pop, b->vel, b->stream);
@@ -2456,6 +2481,7 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
     }
     hipFree(s.pop);
     hipFree(s.vel);
+    b->smode = ok ? bestMode : smode0;
     if (!lbmbatch_set_block_size(b, ok ? best : bs0)) return 0;
     return ok ? best : 0;
 }

@@ -109,6 +109,8 @@ SIGNATURES = {
     "LBM_BatchProfile": (c_int, [c_void_p, c_int, c_int, c_float_p]),
     "LBM_BatchAutotune": (c_int, [c_void_p, c_int, c_float_p]),
     "LBM_BatchSetBlockSize": (c_int, [c_void_p, c_int]),
+    "LBM_BatchSetStreamingAccess": (c_int, [c_void_p, c_int]),
+    "LBM_BatchGetKernelConfig": (c_int, [c_void_p, c_int_p, c_int_p]),
     "LBM_BatchTunnelCount": (c_int, [c_void_p]),
     "LBM_BatchTotalCells": (ctypes.c_longlong, [c_void_p]),
     "LBM_BatchComputeForcesF64": (c_int, [c_void_p, c_double_p, ctypes.POINTER(ctypes.c_longlong)]),
@@ -505,6 +507,17 @@ class Batch:
     def set_block_size(self, bs: int):
         if not load_library().LBM_BatchSetBlockSize(self.ptr, bs):
             raise RuntimeError("invalid block size")
+
+    def set_streaming(self, mode: int):
+        if not load_library().LBM_BatchSetStreamingAccess(self.ptr, int(mode)):
+            raise ValueError("invalid streaming mode (0-3) or batch not initialised")
+
+    def kernel_config(self):
+        """(workgroup size, streaming mode) of the batch step kernel."""
+        bs, nt = c_int(), c_int()
+        if not load_library().LBM_BatchGetKernelConfig(self.ptr, ctypes.byref(bs), ctypes.byref(nt)):
+            raise RuntimeError("batch not initialised")
+        return bs.value, nt.value
 
     @property
     def ntunnels(self) -> int:

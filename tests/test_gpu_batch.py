@@ -1,7 +1,7 @@
 """Batched tunnels: every tunnel of a batch evolves bit-for-bit as the same
 tunnel run alone through LBM_Step (which the parity tests tie to the oracle),
-with and without HIP-graph replay, for every workgroup size the autotuner
-can pick."""
+with and without HIP-graph replay, for every workgroup size and streaming
+mode the autotuner can pick."""
 import numpy as np
 import pytest
 
@@ -68,14 +68,19 @@ def test_graph_replay_autotune_and_block_sizes(gpu):
     for _ in range(24):  # eager single steps, no graph
         gpu.load_library().LBM_BatchStep(ref.ptr, 1)
     results = []
-    for bs in (128, 512):
+    for bs, mode in ((128, 0), (512, 0), (256, 1), (512, 3), (128, 2)):
         b = build()
         b.set_block_size(bs)
+        b.set_streaming(mode)  # cache policy only: the fields stay bit-identical
+        assert b.kernel_config() == (bs, mode)
         b.step(24)
         results.append(b)
+    with pytest.raises(ValueError):
+        results[0].set_streaming(4)
     tuned = build()
     best, ms = tuned.autotune(4)
     assert best in (128, 256, 512) and np.all(ms > 0)
+    assert tuned.kernel_config()[0] == best and tuned.kernel_config()[1] in (0, 1, 3)
     tuned.step(24)  # tuning restored the state, so this is still steps 1..24
     results.append(tuned)
     for b in results:
