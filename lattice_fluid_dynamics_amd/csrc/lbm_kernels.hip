@@ -2488,17 +2488,13 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
 
 // ------------------------------------------------------ copy bandwidth probe
 // A plain streaming copy used to measure the attainable HBM bandwidth and
-// to calibrate byte counters: each lane moves four 16-byte vectors, one
-// workgroup per 16 KiB, no grid-stride loop.
+// to calibrate byte counters: one 16-byte vector per lane, one load and one
+// store, no grid-stride loop.  Of the copy shapes measured on MI355X this is
+// the fastest (6.1-6.2 TB/s on 2.5 GB; four vectors per lane reach 5.55,
+// 19 separate planes of 4-byte elements 5.5).
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n) {
-    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
-    float4 v[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (base + 256 * k < n) v[k] = src[base + 256 * k];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (base + 256 * k < n) dst[base + 256 * k] = v[k];
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 extern "C" int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float *ms_per_copy) {
@@ -2512,7 +2508,7 @@ extern "C" int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float
     }
     hipMemset(a, 0, (size_t)n * 16);
     hipMemset(b, 0, (size_t)n * 16);
-    const int grid = (int)((n + 1023) / 1024);
+    const unsigned grid = (unsigned)((n + 255) / 256);
     hipEvent_t t0, t1;
     hipEventCreate(&t0);
     hipEventCreate(&t1);
