@@ -526,6 +526,23 @@ struct StepArgs {
 constexpr int XPLUS[5] = {1, 7, 9, 11, 13};
 constexpr int XMINUS[5] = {2, 8, 10, 12, 14};
 
+// The value of lane l - 1 (UP) or l + 1 (!UP) of the 64-lane wave, through a
+// whole-wave DPP shift (one VALU instruction, no LDS traffic).  The lane with
+// no source (0 for UP, 63 otherwise) keeps its own value.
+template <bool UP>
+__device__ __forceinline__ float wave_shift1(float v) {
+    const int i = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(i, i, UP ? 0x138 : 0x130, 0xF, 0xF, false));
+}
+template <bool UP>
+__device__ __forceinline__ double wave_shift1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)b, hi = (int)(b >> 32);
+    const int slo = __builtin_amdgcn_update_dpp(lo, lo, UP ? 0x138 : 0x130, 0xF, 0xF, false);
+    const int shi = __builtin_amdgcn_update_dpp(hi, hi, UP ? 0x138 : 0x130, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)shi << 32) | (unsigned)slo);
+}
+
 template <int FMT, int OP, int PHASE, int ROW = 0, int NT = 0>
 __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell) {
     typedef Pop<FMT> P;
@@ -591,7 +608,8 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         // slot (i, x0 + l): for ex_i = +1 the value of cell x0 + l - 1 (the
         // lane below, or lane 63 of the previous wave of the row, wrapping),
         // for ex_i = -1 that of cell x0 + l + 1; every store then covers
-        // whole cache lines.
+        // whole cache lines.  Values move between lanes by whole-wave DPP
+        // shifts; the wave-edge values come through LDS.
         __shared__ T xp[8][5], xm[8][5];
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         const int wpr = g.nx >> 6, rw = w % wpr, w0 = w - rw;
@@ -607,13 +625,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
             const uint32_t dst = (uint32_t)(X[1] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
             T v = fo[i];
             if (EX[i] == 1) {
-                v = __shfl_up(v, 1, 64);
+                v = wave_shift1<true>(v);
                 int k = 0;
 #pragma unroll
                 for (int j = 0; j < 5; j++) k = XPLUS[j] == i ? j : k;
                 if (lane == 0) v = xp[wprev][k];
             } else if (EX[i] == -1) {
-                v = __shfl_down(v, 1, 64);
+                v = wave_shift1<false>(v);
                 int k = 0;
 #pragma unroll
                 for (int j = 0; j < 5; j++) k = XMINUS[j] == i ? j : k;
