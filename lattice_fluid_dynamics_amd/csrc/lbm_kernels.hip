@@ -901,14 +901,33 @@ __global__ __launch_bounds__(64) void k_finalize(const double *pd, const long lo
     __shared__ double sd[64][NV > 0 ? NV : 1];
     __shared__ long long si[64][NI > 0 ? NI : 1];
     const int l = threadIdx.x;
+    // lane l adds partials l, l + 64, l + 128, ... in that order; 16 loads are
+    // in flight before their adds (the order of the adds is unchanged)
+    constexpr int U = 16;
     for (int k = 0; k < NV; k++) {
         double s = 0;
-        for (int b = l; b < nblocks; b += 64) s += pd[(long long)b * NV + k];
+        int b = l;
+        for (; b + 64 * (U - 1) < nblocks; b += 64 * U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = pd[(long long)(b + 64 * u) * NV + k];
+#pragma unroll
+            for (int u = 0; u < U; u++) s += v[u];
+        }
+        for (; b < nblocks; b += 64) s += pd[(long long)b * NV + k];
         sd[l][k] = s;
     }
     for (int k = 0; k < NI; k++) {
         long long s = 0;
-        for (int b = l; b < nblocks; b += 64) s += pi[(long long)b * NI + k];
+        int b = l;
+        for (; b + 64 * (U - 1) < nblocks; b += 64 * U) {
+            long long v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = pi[(long long)(b + 64 * u) * NI + k];
+#pragma unroll
+            for (int u = 0; u < U; u++) s += v[u];
+        }
+        for (; b < nblocks; b += 64) s += pi[(long long)b * NI + k];
         si[l][k] = s;
     }
     __syncthreads();
