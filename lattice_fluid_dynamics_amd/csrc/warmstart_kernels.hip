@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, 
 // finishes the result.
 constexpr int FB_LD = 40;    // bf16 per staged halo position
 constexpr int FB_HALO = 360; // 10 x 6 x 6
+constexpr int FB_IT = (FB_HALO * 4 + 255) / 256;  // halo loads per lane and chunk
 
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
@@ -167,16 +168,32 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    // halo of one 32-channel chunk: FB_IT 16-byte loads per lane, all issued
+    // before any is written to LDS; the next chunk's are in flight while the
+    // current chunk's taps run
+    auto fetch = [&](int c0, uint4 (&hv)[FB_IT]) {
+#pragma unroll
+        for (int k = 0; k < FB_IT; k++) {
+            const int e = tid + 256 * k;
+            hv[k] = make_uint4(0, 0, 0, 0);
+            if (e < FB_HALO * 4) {
+                const int hp = e >> 2, cg = e & 3;
+                const int X_ = x0 + hp % 10 - 1, Y_ = y0 + (hp / 10) % 6 - 1, Z_ = z0 + hp / 60 - 1;
+                if (X_ >= 0 && X_ < s.w && Y_ >= 0 && Y_ < s.h && Z_ >= 0 && Z_ < s.d)
+                    hv[k] = *(const uint4 *)(X + (sbase + ((long long)Z_ * s.h + Y_) * s.w + X_) * cin + c0 + 8 * cg);
+            }
+        }
+    };
+    uint4 hv[FB_IT];
+    fetch(0, hv);
     for (int c0 = 0; c0 < cin; c0 += 32) {
-        for (int e = tid; e < FB_HALO * 4; e += 256) {
-            const int hp = e >> 2, cg = e & 3;
-            const int X_ = x0 + hp % 10 - 1, Y_ = y0 + (hp / 10) % 6 - 1, Z_ = z0 + hp / 60 - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (X_ >= 0 && X_ < s.w && Y_ >= 0 && Y_ < s.h && Z_ >= 0 && Z_ < s.d)
-                v = *(const uint4 *)(X + (sbase + ((long long)Z_ * s.h + Y_) * s.w + X_) * cin + c0 + 8 * cg);
-            *(uint4 *)(halo + hp * FB_LD + 8 * cg) = v;
+#pragma unroll
+        for (int k = 0; k < FB_IT; k++) {
+            const int e = tid + 256 * k;
+            if (e < FB_HALO * 4) *(uint4 *)(halo + (e >> 2) * FB_LD + 8 * (e & 3)) = hv[k];
         }
         __syncthreads();
+        if (c0 + 32 < cin) fetch(c0 + 32, hv);
         // per tap: two A reads from the halo, two B reads of weight rows, four
         // MFMAs; the operands of tap t + 1 are loaded before tap t's MFMAs
         auto load = [&](int t, Pack8 (&a)[2], Pack8 (&b)[2]) {
