@@ -87,7 +87,15 @@ int LBM_WarmStartInitialize(LBMGrid *grid, LBMWarmStart *ws, float u);
  * channels-last BF16 unless noted).  1 on success. */
 int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, float *Yf, int n, int d, int h, int w,
                    int cin, int cout, int relu);
+/* As lbmnn_conv_fwd, with Wp the packed copy of W from lbmnn_conv_pack
+ * (used where the extent divides into 8x4x4 bricks; W elsewhere). */
+int lbmnn_conv_fwd_packed(const void *X, const void *W, const void *Wp, const float *bias, void *Y, float *Yf, int n,
+                          int d, int h, int w, int cin, int cout, int relu);
 int lbmnn_conv_mirror(const void *W, void *Wm, int cin, int cout);
+/* From W[co][t][ci] (cin, cout % 32 == 0): Wm = lbmnn_conv_mirror's output,
+ * Wp = W packed for lbmnn_conv_fwd_packed, Wmp = Wm packed (its convolution
+ * has cin and cout swapped).  Wp and Wmp hold as many elements as W. */
+int lbmnn_conv_pack(const void *W, void *Wm, void *Wp, void *Wmp, int cin, int cout);
 /* dW += weight gradient and (db != NULL) db += bias gradient; cin, cout % 32 == 0 */
 int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n, int d, int h, int w, int cin,
                      int cout);

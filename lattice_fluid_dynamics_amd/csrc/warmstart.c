@@ -37,6 +37,7 @@ static const int ACT_C[NACT] = {32, 32, 32, 64, 64, 64, 128, 128, 128, 256, 256,
 typedef struct {
     float *w, *b;   /* FP32 master weights (cout x 27 x cin) and bias */
     void *wb, *wm;  /* BF16 copy and its tap-mirrored transpose (dgrad) */
+    void *wp, *wmp; /* wb and wm in the packed layout of the brick forward */
     float *gw, *gb; /* gradients */
     float *mw, *vw, *mb, *vb;
     long long nw;
@@ -94,7 +95,7 @@ void LBM_WarmStartFree(LBMWarmStart *ws) {
     if (!ws) return;
     for (int k = 0; k < NCONV; k++) {
         Conv *c = &ws->conv[k];
-        void *ps[] = {c->w, c->b, c->wb, c->wm, c->gw, c->gb, c->mw, c->vw, c->mb, c->vb};
+        void *ps[] = {c->w, c->b, c->wb, c->wm, c->wp, c->wmp, c->gw, c->gb, c->mw, c->vw, c->mb, c->vb};
         for (size_t j = 0; j < sizeof(ps) / sizeof(ps[0]); j++)
             if (ps[j]) lbmdev_free(ps[j]);
     }
@@ -114,7 +115,7 @@ void LBM_WarmStartFree(LBMWarmStart *ws) {
 
 static int refresh_bf16(LBMWarmStart *ws, int k) {
     Conv *c = &ws->conv[k];
-    return lbmnn_f32_to_bf16(c->w, c->wb, c->nw) && lbmnn_conv_mirror(c->wb, c->wm, CIN[k], COUT[k]);
+    return lbmnn_f32_to_bf16(c->w, c->wb, c->nw) && lbmnn_conv_pack(c->wb, c->wm, c->wp, c->wmp, CIN[k], COUT[k]);
 }
 
 LBMWarmStart *LBM_WarmStartCreate(int cnx, int cny, int cnz, int maxBatch, unsigned int seed) {
@@ -141,6 +142,8 @@ LBMWarmStart *LBM_WarmStartCreate(int cnx, int cny, int cnz, int maxBatch, unsig
         DALLOC(c->w, wbytes);
         DALLOC(c->b, bbytes);
         DALLOC(c->wb, 2 * (size_t)c->nw);
+        DALLOC(c->wp, 2 * (size_t)c->nw);
+        DALLOC(c->wmp, 2 * (size_t)c->nw);
         DALLOC(c->wm, 2 * (size_t)c->nw);
         DALLOC(c->gw, wbytes);
         DALLOC(c->gb, bbytes);
@@ -205,7 +208,8 @@ long long LBM_WarmStartParamCount(const LBMWarmStart *ws) {
 /* ------------------------------------------------------------ forward */
 static int conv_fwd(LBMWarmStart *ws, int k, const void *in, void *out, float *outf, int B, int relu) {
     const int *d = ws->dims[LEVEL[k]];
-    return lbmnn_conv_fwd(in, ws->conv[k].wb, ws->conv[k].b, out, outf, B, d[0], d[1], d[2], CIN[k], COUT[k], relu);
+    const Conv *c = &ws->conv[k];
+    return lbmnn_conv_fwd_packed(in, c->wb, c->wp, c->b, out, outf, B, d[0], d[1], d[2], CIN[k], COUT[k], relu);
 }
 
 static int forward(LBMWarmStart *ws, const void *x0, int B, float *yf) {
@@ -247,7 +251,8 @@ static int conv_bwd(LBMWarmStart *ws, int k, void *gout, const void *in, void *g
     int ok = 1;
     if (relu) ok = ok && lbmnn_relu_bwd(gout, out, P * COUT[k]);
     ok = ok && lbmnn_conv_wgrad(gout, in, c->gw, c->gb, B, d[0], d[1], d[2], CIN[k], COUT[k]);
-    if (gin) ok = ok && lbmnn_conv_fwd(gout, c->wm, NULL, gin, NULL, B, d[0], d[1], d[2], COUT[k], CIN[k], 0);
+    if (gin)
+        ok = ok && lbmnn_conv_fwd_packed(gout, c->wm, c->wmp, NULL, gin, NULL, B, d[0], d[1], d[2], COUT[k], CIN[k], 0);
     return ok;
 }
 
@@ -332,7 +337,7 @@ float LBM_WarmStartTrainStep(LBMWarmStart *ws, const void *input, const float *t
         Conv *c = &ws->conv[k];
         ok = lbmnn_adamw(c->w, c->wb, c->gw, c->mw, c->vw, c->nw, lr, 0.9f, 0.999f, 1e-8f, weightDecay, ws->step) &&
              lbmnn_adamw(c->b, NULL, c->gb, c->mb, c->vb, COUT[k], lr, 0.9f, 0.999f, 1e-8f, 0.0f, ws->step) &&
-             lbmnn_conv_mirror(c->wb, c->wm, CIN[k], COUT[k]);
+             lbmnn_conv_pack(c->wb, c->wm, c->wp, c->wmp, CIN[k], COUT[k]);
     }
     if (!ok || !lbmnn_synchronize()) {
         ws_err("LBM_WarmStartTrainStep: device error");

@@ -141,7 +141,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, 
 // 32 positions of brick plane z = w (M = 2 x 16: x fastest, two y rows per
 // block) against the 32 output channels (N = 2 x 16) through all 27 taps:
 // per tap two A reads from the halo (16 bytes: 8 channels of the tap-shifted
-// position) and two B reads of the weight rows (L1/L2-resident), four MFMAs.
+// position) and two B reads of the weights, four MFMAs.  The weights come in
+// the packed layout of k_conv_pack: each B read of a wave is 1 KiB
+// contiguous (eight whole cache lines) instead of 64 bytes from each of 16
+// rows 1.7 KiB apart, which made the weight reads the larger part of the
+// kernel's time (level 0: 20.3 us against 14.1 us packed).
 // The input is read once per brick plus halo instead of once per tap.
 // Small grids split the taps over gridDim.z slices (as k_conv_fwd does):
 // slice z writes its FP32 partial sum to part[z][p][co] and k_conv_epilogue
@@ -151,7 +155,7 @@ constexpr int FB_HALO = 360; // 10 x 6 x 6
 constexpr int FB_IT = (FB_HALO * 4 + 255) / 256;  // halo loads per lane and chunk
 
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
+__global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict__ X, const uint4 *__restrict__ Wp,
                                                         const float *__restrict__ bias, bf16_t *__restrict__ Y,
                                                         float *__restrict__ Yf, float *__restrict__ part, Shape s,
                                                         int cin, int cout, int act) {
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_brick(const bf16_t *__restrict
             }
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
-                b[nb].u = *(const uint4 *)(Wt + ((long long)(co0 + 16 * nb + r) * 27 + t) * cin + c0 + 8 * kq);
+                b[nb].u = Wp[(((long long)(blockIdx.y * (cin / 32) + c0 / 32) * 27 + t) * 2 + nb) * 64 + lane];
         };
         auto mma = [&](const Pack8 (&a)[2], const Pack8 (&b)[2]) {
 #pragma unroll
@@ -294,6 +298,80 @@ float *part_buffer(size_t bytes) {
     }
     g_part_bytes = bytes;
     return g_part;
+}
+
+// Packed forward weights for calls that pass only W[co][t][ci]: one device
+// buffer, grown on demand (same threading rule as part_buffer).
+uint4 *g_pack = nullptr;
+size_t g_pack_bytes = 0;
+
+uint4 *pack_buffer(size_t bytes) {
+    if (bytes <= g_pack_bytes) return g_pack;
+    if (g_pack) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_pack);
+        g_pack = nullptr;
+        g_pack_bytes = 0;
+    }
+    if (hipMalloc((void **)&g_pack, bytes) != hipSuccess) {
+        g_pack = nullptr;
+        return nullptr;
+    }
+    g_pack_bytes = bytes;
+    return g_pack;
+}
+
+// Packed weights of the brick forward (cin, cout multiples of 32): for
+// output-channel block cb (32 channels), input chunk ch (32 channels), tap t
+// and MFMA column block nb, the 64 uint4 read by the lanes l = 16 kq + r of
+// a wave, lane l holding W[32 cb + 16 nb + r][t][32 ch + 8 kq .. + 8]:
+//   Wp[(((cb * cin / 32 + ch) * 27 + t) * 2 + nb) * 64 + l].
+// k_conv_pack writes, from W[co][t][ci] in one launch: the packed forward
+// weights Wp; the input-gradient weights Wm[ci][26 - t][co] (as
+// k_conv_mirror); and Wm packed (Wmp, a convolution with cin and cout
+// swapped).  Threads [0, n8) take one 8-channel group of W each, threads
+// [n8, 2 n8) one uint4 of Wmp.
+__global__ void k_conv_pack(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wm, uint4 *__restrict__ Wp,
+                            uint4 *__restrict__ Wmp, int cin, int cout) {
+    const long long n8 = (long long)cout * 27 * cin / 8;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n8) {
+        const int ci = (int)(i % (cin / 8)) * 8;
+        const int t = (int)((i / (cin / 8)) % 27);
+        const int co = (int)(i / ((long long)(cin / 8) * 27));
+        const uint4 v = *(const uint4 *)(W + ((long long)co * 27 + t) * cin + ci);
+        const int lane = 16 * ((ci % 32) / 8) + co % 16;
+        Wp[(((long long)(co / 32) * (cin / 32) + ci / 32) * 27 + t) * 128 + ((co % 32) / 16) * 64 + lane] = v;
+        const bf16_t *h = (const bf16_t *)&v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) Wm[((long long)(ci + j) * 27 + (26 - t)) * cout + co] = h[j];
+        return;
+    }
+    i -= n8;
+    if (i >= n8) return;
+    // Wmp of the swapped convolution: output channel o (an input channel of
+    // W), tap tm, input channels c .. c + 8 (output channels of W)
+    const int lane = (int)(i % 64), nb = (int)((i / 64) % 2), tm = (int)((i / 128) % 27);
+    const long long blk = i / (128 * 27);  // = ob * (cout / 32) + ch
+    const int ch = (int)(blk % (cout / 32)), ob = (int)(blk / (cout / 32));
+    const int o = 32 * ob + 16 * nb + (lane % 16), c = 32 * ch + 8 * (lane / 16);
+    uint4 v;
+    bf16_t *h = (bf16_t *)&v;
+#pragma unroll
+    for (int j = 0; j < 8; j++) h[j] = W[((long long)(c + j) * 27 + (26 - tm)) * cin + o];
+    Wmp[i] = v;
+}
+
+// Wp alone (the first half of k_conv_pack), for callers without packed copies.
+__global__ void k_conv_pack_fwd(const bf16_t *__restrict__ W, uint4 *__restrict__ Wp, int cin, int cout) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)cout * 27 * cin / 8) return;
+    const int ci = (int)(i % (cin / 8)) * 8;
+    const int t = (int)((i / (cin / 8)) % 27);
+    const int co = (int)(i / ((long long)(cin / 8) * 27));
+    const int lane = 16 * ((ci % 32) / 8) + co % 16;
+    Wp[(((long long)(co / 32) * (cin / 32) + ci / 32) * 27 + t) * 128 + ((co % 32) / 16) * 64 + lane] =
+        *(const uint4 *)(W + ((long long)co * 27 + t) * cin + ci);
 }
 
 // W[co][t][ci] -> Wm[ci][26 - t][co] (mirror taps, swap channels): the
@@ -813,9 +891,18 @@ static int ws_check(const char *what) {
     return 1;
 }
 
-// cin % 32 == 0, cout % 16 == 0.
-int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, float *Yf, int n, int d, int h, int w,
-                   int cin, int cout, int relu) {
+int lbmnn_conv_pack(const void *W, void *Wm, void *Wp, void *Wmp, int cin, int cout) {
+    if (cin % 32 != 0 || cout % 32 != 0) return 0;
+    const long long n8 = (long long)cout * 27 * cin / 8;
+    hipLaunchKernelGGL(k_conv_pack, g1(2 * n8), dim3(256), 0, 0, (const bf16_t *)W, (bf16_t *)Wm, (uint4 *)Wp,
+                       (uint4 *)Wmp, cin, cout);
+    return ws_check("weight pack");
+}
+
+// cin % 32 == 0, cout % 16 == 0.  Wp: the packed weights of k_conv_pack, or
+// NULL to pack W here when the brick path runs.
+static int conv_fwd_any(const void *X, const void *W, const void *Wp, const float *bias, void *Y, float *Yf, int n,
+                        int d, int h, int w, int cin, int cout, int relu) {
     if (cin % 32 != 0 || cout % 16 != 0) return 0;
     Shape s{n, d, h, w};
     const long long P = (long long)n * d * h * w;
@@ -835,12 +922,22 @@ int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, flo
                 return 0;
             }
         }
+        if (!Wp) {  // pack W here (the network keeps packed copies instead)
+            const long long n8 = (long long)cout * 27 * cin / 8;
+            uint4 *buf = pack_buffer(sizeof(uint4) * (size_t)n8);
+            if (!buf) {
+                fprintf(stderr, "warm-start conv forward: weight pack buffer allocation failed\n");
+                return 0;
+            }
+            hipLaunchKernelGGL(k_conv_pack_fwd, g1(n8), dim3(256), 0, 0, (const bf16_t *)W, buf, cin, cout);
+            Wp = buf;
+        }
         dim3 grid((unsigned)(P / 128), (unsigned)(cout / 32), (unsigned)nsplit);
         if (part)
-            hipLaunchKernelGGL(k_conv_fwd_brick<true>, grid, dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)W,
+            hipLaunchKernelGGL(k_conv_fwd_brick<true>, grid, dim3(256), 0, 0, (const bf16_t *)X, (const uint4 *)Wp,
                                bias, (bf16_t *)Y, Yf, part, s, cin, cout, relu);
         else
-            hipLaunchKernelGGL(k_conv_fwd_brick<false>, grid, dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)W,
+            hipLaunchKernelGGL(k_conv_fwd_brick<false>, grid, dim3(256), 0, 0, (const bf16_t *)X, (const uint4 *)Wp,
                                bias, (bf16_t *)Y, Yf, part, s, cin, cout, relu);
         if (part) {
             const long long m = P * cout;
@@ -877,6 +974,16 @@ int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, flo
         hipLaunchKernelGGL(k_conv_epilogue, g1(m), dim3(256), 0, 0, part, nsplit, bias, y, Yf, m, cout, relu);
     }
     return ws_check("conv forward");
+}
+
+int lbmnn_conv_fwd(const void *X, const void *W, const float *bias, void *Y, float *Yf, int n, int d, int h, int w,
+                   int cin, int cout, int relu) {
+    return conv_fwd_any(X, W, nullptr, bias, Y, Yf, n, d, h, w, cin, cout, relu);
+}
+
+int lbmnn_conv_fwd_packed(const void *X, const void *W, const void *Wp, const float *bias, void *Y, float *Yf, int n,
+                          int d, int h, int w, int cin, int cout, int relu) {
+    return conv_fwd_any(X, W, Wp, bias, Y, Yf, n, d, h, w, cin, cout, relu);
 }
 
 int lbmnn_conv_mirror(const void *W, void *Wm, int cin, int cout) {

@@ -46,7 +46,9 @@ WS_SIGNATURES = {
     "LBM_CoarseSolidFraction": (c_int, [lbm.GridP, c_int, c_float_p]),
     "LBM_WarmStartInitialize": (c_int, [lbm.GridP, c_void_p, c_float]),
     "lbmnn_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 7),
+    "lbmnn_conv_fwd_packed": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 7),
     "lbmnn_conv_mirror": (c_int, [c_void_p, c_void_p, c_int, c_int]),
+    "lbmnn_conv_pack": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "lbmnn_conv_wgrad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 6),
     "lbmnn_relu_bwd": (c_int, [c_void_p, c_void_p, ctypes.c_longlong]),
     "lbmnn_bias_grad": (c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_int]),

@@ -83,6 +83,61 @@ def test_conv_forward_and_dgrad(gpu, shape):
         assert rel_l2(DXf.download(np.float32, (n, d, h, w, cin)), C.conv_dgrad(Gv, Wv)) < 1e-6
 
 
+def packed(w):
+    """The brick forward's weight layout of a (cout, 27, cin) array: index
+    (((cb * cin / 32 + ch) * 27 + t) * 2 + nb) * 64 + 16 kq + r holds the 8
+    channels ci = 32 ch + 8 kq + [0, 8) of output channel co = 32 cb + 16 nb + r."""
+    cout, _, cin = w.shape
+    a = w.reshape(cout // 32, 2, 16, 27, cin // 32, 4, 8)  # cb, nb, r, t, ch, kq, j
+    return np.ascontiguousarray(a.transpose(0, 4, 3, 1, 5, 2, 6)).reshape(-1)
+
+
+@pytest.mark.parametrize("cin,cout", [(32, 32), (64, 32), (32, 96), (128, 64)])
+def test_weight_pack_and_packed_forward(gpu, cin, cout):
+    """lbmnn_conv_pack: Wm equals lbmnn_conv_mirror's, Wp and Wmp follow the
+    packed layout (checked against a reshape of W and of Wm on the host), and
+    the brick forward and input gradient give the same bits from the packed
+    copies as from W and Wm."""
+    L = W.lib()
+    rng = np.random.default_rng(cin + 7 * cout)
+    Wt, Wv = dev_bf16(rng.normal(size=(cout, 27, cin)) * 0.1)
+    n = cout * 27 * cin
+    Wm, Wp, Wmp, Wm_ref = (dev_zeros(n, 2) for _ in range(4))
+    assert L.lbmnn_conv_pack(Wt.ptr, Wm.ptr, Wp.ptr, Wmp.ptr, cin, cout)
+    assert L.lbmnn_conv_mirror(Wt.ptr, Wm_ref.ptr, cin, cout)
+    bits = C.bf16_bits(Wv).reshape(cout, 27, cin)
+    wm = Wm.download(np.uint16, (cin, 27, cout))
+    assert np.array_equal(wm, Wm_ref.download(np.uint16, (cin, 27, cout)))
+    assert np.array_equal(wm, bits[:, ::-1, :].transpose(2, 1, 0))
+    assert np.array_equal(Wp.download(np.uint16, n), packed(bits))
+    assert np.array_equal(Wmp.download(np.uint16, n), packed(wm))
+    nn_, d, h, w = 2, 4, 8, 16
+    P = nn_ * d * h * w
+    X, _ = dev_bf16(rng.normal(size=(P, cin)))
+    b = W.DeviceArray.from_host(rng.normal(size=cout).astype(np.float32))
+    out = []
+    for wp in (None, Wp):
+        Y, Yf = dev_zeros(P * cout, 2), dev_zeros(P * cout, F32)
+        if wp is None:
+            assert L.lbmnn_conv_fwd(X.ptr, Wt.ptr, b.ptr, Y.ptr, Yf.ptr, nn_, d, h, w, cin, cout, 1)
+        else:
+            assert L.lbmnn_conv_fwd_packed(X.ptr, Wt.ptr, wp.ptr, b.ptr, Y.ptr, Yf.ptr, nn_, d, h, w, cin, cout, 1)
+        out.append((Y.download(np.uint16, P * cout), Yf.download(np.float32, P * cout)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    G, _ = dev_bf16(rng.normal(size=(P, cout)))
+    grads = []
+    for wm_, wmp in ((Wm, None), (Wm, Wmp)):
+        DX = dev_zeros(P * cin, F32)
+        if wmp is None:
+            assert L.lbmnn_conv_fwd(G.ptr, wm_.ptr, None, dev_zeros(P * cin, 2).ptr, DX.ptr, nn_, d, h, w, cout,
+                                    cin, 0)
+        else:
+            assert L.lbmnn_conv_fwd_packed(G.ptr, wm_.ptr, wmp.ptr, None, dev_zeros(P * cin, 2).ptr, DX.ptr, nn_, d,
+                                           h, w, cout, cin, 0)
+        grads.append(DX.download(np.float32, P * cin))
+    assert np.array_equal(grads[0], grads[1])
+
+
 @pytest.mark.parametrize("shape", WGRAD_SHAPES)
 def test_conv_weight_gradient(gpu, shape):
     n, d, h, w, cin, cout = shape
