@@ -65,11 +65,13 @@ union Pack8 {
 // (split-K for the small deep levels): slice z writes its FP32 partial sum to
 // part[z][p][co] and k_conv_epilogue finishes the result.
 // act: 0 none, 1 ReLU.  Yf (optional) receives the FP32 result.
-template <int NB>
+// PK: the weights come packed (k_conv_pack layout, cout % 32 == 0) in Wp,
+// so each B read of a wave is 1 KiB contiguous; otherwise rows of Wt.
+template <int NB, bool PK>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, const bf16_t *__restrict__ Wt,
-                                                  const float *__restrict__ bias, bf16_t *__restrict__ Y,
-                                                  float *__restrict__ Yf, float *__restrict__ part, Shape s,
-                                                  int cin, int cout, int act) {
+                                                  const uint4 *__restrict__ Wp, const float *__restrict__ bias,
+                                                  bf16_t *__restrict__ Y, float *__restrict__ Yf,
+                                                  float *__restrict__ part, Shape s, int cin, int cout, int act) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int P = s.n * s.d * s.h * s.w;
     const int p0 = (blockIdx.x * 4 + wave) * 32;
@@ -101,7 +103,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t *__restrict__ X, 
             for (int b = 0; b < NB; b++) {
                 const int co = co0 + 16 * b + r;
                 B[b].u = make_uint4(0, 0, 0, 0);
-                if (co < cout) B[b].u = *(const uint4 *)(Wt + ((long long)co * 27 + t) * cin + c0 + kq * 8);
+                if (co < cout) {
+                    if (PK) {
+                        const int blk16 = co0 / 16 + b;
+                        B[b].u = Wp[(((long long)(blk16 >> 1) * (cin / 32) + c0 / 32) * 27 + t) * 128 +
+                                    (blk16 & 1) * 64 + lane];
+                    } else {
+                        B[b].u = *(const uint4 *)(Wt + ((long long)co * 27 + t) * cin + c0 + kq * 8);
+                    }
+                }
             }
 #pragma unroll
             for (int a = 0; a < 2; a++)
@@ -899,8 +909,9 @@ int lbmnn_conv_pack(const void *W, void *Wm, void *Wp, void *Wmp, int cin, int c
     return ws_check("weight pack");
 }
 
-// cin % 32 == 0, cout % 16 == 0.  Wp: the packed weights of k_conv_pack, or
-// NULL to pack W here when the brick path runs.
+// cin % 32 == 0, cout % 16 == 0.  Wp: the packed weights of k_conv_pack
+// (read by both paths), or NULL: the brick path then packs W into a scratch
+// buffer and the per-tile path reads W.
 static int conv_fwd_any(const void *X, const void *W, const void *Wp, const float *bias, void *Y, float *Yf, int n,
                         int d, int h, int w, int cin, int cout, int relu) {
     if (cin % 32 != 0 || cout % 16 != 0) return 0;
@@ -962,12 +973,22 @@ static int conv_fwd_any(const void *X, const void *W, const void *Wp, const floa
     }
     dim3 grid((unsigned)((P + 127) / 128), (unsigned)((nbTotal + NB - 1) / NB), (unsigned)nsplit);
     const bf16_t *x = (const bf16_t *)X, *wt = (const bf16_t *)W;
+    const uint4 *wp = (const uint4 *)Wp;
     bf16_t *y = (bf16_t *)Y;
-    switch (NB) {
-    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
-    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
-    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
-    default: hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, 0, x, wt, bias, y, Yf, part, s, cin, cout, relu); break;
+    if (wp && cout % 32 == 0) {
+        switch (NB) {
+        case 1: hipLaunchKernelGGL((k_conv_fwd<1, true>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        case 2: hipLaunchKernelGGL((k_conv_fwd<2, true>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        case 3: hipLaunchKernelGGL((k_conv_fwd<3, true>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        default: hipLaunchKernelGGL((k_conv_fwd<4, true>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        }
+    } else {
+        switch (NB) {
+        case 1: hipLaunchKernelGGL((k_conv_fwd<1, false>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        case 2: hipLaunchKernelGGL((k_conv_fwd<2, false>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        case 3: hipLaunchKernelGGL((k_conv_fwd<3, false>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        default: hipLaunchKernelGGL((k_conv_fwd<4, false>), grid, dim3(256), 0, 0, x, wt, wp, bias, y, Yf, part, s, cin, cout, relu); break;
+        }
     }
     if (part) {
         const long long m = P * cout;

@@ -96,8 +96,9 @@ def packed(w):
 def test_weight_pack_and_packed_forward(gpu, cin, cout):
     """lbmnn_conv_pack: Wm equals lbmnn_conv_mirror's, Wp and Wmp follow the
     packed layout (checked against a reshape of W and of Wm on the host), and
-    the brick forward and input gradient give the same bits from the packed
-    copies as from W and Wm."""
+    the forward and input gradient give the same bits from the packed copies
+    as from W and Wm, on a brick-tiled extent (2x4x8x16) and on one that is
+    not (3x2x2x4, per-tile kernels)."""
     L = W.lib()
     rng = np.random.default_rng(cin + 7 * cout)
     Wt, Wv = dev_bf16(rng.normal(size=(cout, 27, cin)) * 0.1)
@@ -111,7 +112,11 @@ def test_weight_pack_and_packed_forward(gpu, cin, cout):
     assert np.array_equal(wm, bits[:, ::-1, :].transpose(2, 1, 0))
     assert np.array_equal(Wp.download(np.uint16, n), packed(bits))
     assert np.array_equal(Wmp.download(np.uint16, n), packed(wm))
-    nn_, d, h, w = 2, 4, 8, 16
+    for nn_, d, h, w in ((2, 4, 8, 16), (3, 2, 2, 4)):
+        packed_forward_matches(L, rng, Wt, Wp, Wm, Wmp, nn_, d, h, w, cin, cout)
+
+
+def packed_forward_matches(L, rng, Wt, Wp, Wm, Wmp, nn_, d, h, w, cin, cout):
     P = nn_ * d * h * w
     X, _ = dev_bf16(rng.normal(size=(P, cin)))
     b = W.DeviceArray.from_host(rng.normal(size=cout).astype(np.float32))
