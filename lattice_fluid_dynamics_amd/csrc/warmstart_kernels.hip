@@ -428,30 +428,56 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t *__restrict__ d
 #pragma unroll
     for (int b = 0; b < PER_WAVE; b++) acc[b] = f32x4{0, 0, 0, 0};
     float bsum = 0.0f;
-    for (int p0 = pbeg; p0 < pend; p0 += WG_K) {
-        // dZ tile: WG_K positions x TCO channels, TCO/8 lanes per position
-        for (int e = tid; e < WG_K * TCO / 8; e += 256) {
+    // one slab: WG_K positions of dZ (TCO channels) and of X at tap t (TCI
+    // channels), 16-byte loads of 8 channels; the next slab's loads are in
+    // flight while the current slab's MFMAs run
+    constexpr int ZN = (WG_K * TCO / 8 + 255) / 256, XN = (WG_K * TCI / 8 + 255) / 256;
+    auto fetch = [&](int p0, uint4 (&vz)[ZN], uint4 (&vx)[XN]) {
+#pragma unroll
+        for (int k = 0; k < ZN; k++) {
+            const int e = tid + 256 * k;
             const int pl = e / (TCO / 8), cg = e % (TCO / 8);
             const int p = p0 + pl;
-            Pack8 v;
-            v.u = make_uint4(0, 0, 0, 0);
-            if (p < pend) v.u = *(const uint4 *)(dZ + (long long)p * cout + co0 + 8 * cg);
-            const bf16_t *h = (const bf16_t *)&v.u;
-#pragma unroll
-            for (int j = 0; j < 8; j++) Ds[(8 * cg + j) * WG_LD + pl] = h[j];
+            vz[k] = make_uint4(0, 0, 0, 0);
+            if (e < WG_K * TCO / 8 && p < pend) vz[k] = *(const uint4 *)(dZ + (long long)p * cout + co0 + 8 * cg);
         }
-        for (int e = tid; e < WG_K * TCI / 8; e += 256) {
+#pragma unroll
+        for (int k = 0; k < XN; k++) {
+            const int e = tid + 256 * k;
             const int pl = e / (TCI / 8), cg = e % (TCI / 8);
             const int p = p0 + pl;
-            Pack8 v;
-            v.u = make_uint4(0, 0, 0, 0);
+            vx[k] = make_uint4(0, 0, 0, 0);
             int q;
-            if (p < pend && tap_pos(s, p, t, q)) v.u = *(const uint4 *)(X + (long long)q * cin + ci0 + 8 * cg);
-            const bf16_t *h = (const bf16_t *)&v.u;
+            if (e < WG_K * TCI / 8 && p < pend && tap_pos(s, p, t, q))
+                vx[k] = *(const uint4 *)(X + (long long)q * cin + ci0 + 8 * cg);
+        }
+    };
+    uint4 vz[ZN], vx[XN];
+    if (pbeg < pend) fetch(pbeg, vz, vx);
+    for (int p0 = pbeg; p0 < pend; p0 += WG_K) {
+        // transposed into LDS: channel-major rows, positions contiguous
 #pragma unroll
-            for (int j = 0; j < 8; j++) Xs[(8 * cg + j) * WG_LD + pl] = h[j];
+        for (int k = 0; k < ZN; k++) {
+            const int e = tid + 256 * k;
+            if (e < WG_K * TCO / 8) {
+                const int pl = e / (TCO / 8), cg = e % (TCO / 8);
+                const bf16_t *h = (const bf16_t *)&vz[k];
+#pragma unroll
+                for (int j = 0; j < 8; j++) Ds[(8 * cg + j) * WG_LD + pl] = h[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < XN; k++) {
+            const int e = tid + 256 * k;
+            if (e < WG_K * TCI / 8) {
+                const int pl = e / (TCI / 8), cg = e % (TCI / 8);
+                const bf16_t *h = (const bf16_t *)&vx[k];
+#pragma unroll
+                for (int j = 0; j < 8; j++) Xs[(8 * cg + j) * WG_LD + pl] = h[j];
+            }
         }
         __syncthreads();
+        if (p0 + WG_K < pend) fetch(p0 + WG_K, vz, vx);
 #pragma unroll
         for (int b = 0; b < PER_WAVE; b++) {
             const int blk = wave + 4 * b;
@@ -1044,8 +1070,10 @@ int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n,
     }
     const int tco = cout % 64 == 0 ? 64 : 32, tci = cin % 64 == 0 ? 64 : 32;
     const int zb = (cout / tco) * (cin / tci);
-    // about 2048 workgroups, chunks a multiple of the 32-position slab
-    long long chunks = (2048 + 27LL * zb - 1) / (27LL * zb);
+    // about 512 workgroups (a few slabs each, so the next slab's loads overlap
+    // the current MFMAs and fewer partial sums meet in the atomics), chunks a
+    // multiple of the 32-position slab
+    long long chunks = (512 + 27LL * zb - 1) / (27LL * zb);
     long long chunk = (P + chunks - 1) / chunks;
     chunk = ((chunk + WG_K - 1) / WG_K) * WG_K;
     if (chunk < WG_K) chunk = WG_K;
