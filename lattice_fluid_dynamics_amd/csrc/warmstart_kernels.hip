@@ -769,10 +769,50 @@ __global__ void k_zero_list(ZeroList z) {
 
 // ------------------------------------------------------------ elementwise
 // dZ = dY * (Y > 0)  (ReLU backward on the stored activation), in place on dY
+// The elementwise layer kernels below take V = 8 consecutive channels per
+// thread (one 16-byte access) when the channel count allows, else V = 1;
+// every element is computed with the same operations either way.
+template <int V>
+__device__ __forceinline__ void ld_bf(const bf16_t *p, float (&f)[V]) {
+    if constexpr (V == 8) {
+        const uint4 u = *(const uint4 *)p;
+        const bf16_t *h = (const bf16_t *)&u;
+#pragma unroll
+        for (int j = 0; j < 8; j++) f[j] = bf2f(h[j]);
+    } else {
+        f[0] = bf2f(p[0]);
+    }
+}
+template <int V>
+__device__ __forceinline__ void st_bf(bf16_t *p, const float (&f)[V]) {
+    if constexpr (V == 8) {
+        uint4 u;
+        bf16_t *h = (bf16_t *)&u;
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[j] = f2bf(f[j]);
+        *(uint4 *)p = u;
+    } else {
+        p[0] = f2bf(f[0]);
+    }
+}
+
+// dY[i] = 0 where Y[i] <= 0 (bits of the other elements untouched)
+template <int V>
 __global__ void k_relu_bwd(bf16_t *dY, const bf16_t *Y, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * V;
     if (i >= n) return;
-    if (!(bf2f(Y[i]) > 0.0f)) dY[i] = 0;
+    if constexpr (V == 8) {
+        uint4 d = *(const uint4 *)(dY + i);
+        const uint4 y = *(const uint4 *)(Y + i);
+        bf16_t *hd = (bf16_t *)&d;
+        const bf16_t *hy = (const bf16_t *)&y;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (!(bf2f(hy[j]) > 0.0f)) hd[j] = 0;
+        *(uint4 *)(dY + i) = d;
+    } else {
+        if (!(bf2f(Y[i]) > 0.0f)) dY[i] = 0;
+    }
 }
 
 // db[c] += sum over positions of dZ[p][c]
@@ -785,8 +825,9 @@ __global__ void k_bias_grad(const bf16_t *dZ, float *db, long long P, int c) {
 }
 
 // 2x2x2 average pooling: Y (D/2, H/2, W/2) from X
+template <int V>
 __global__ void k_pool_fwd(const bf16_t *X, bf16_t *Y, Shape so, int c) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * V;
     const long long n = (long long)so.n * so.d * so.h * so.w * c;
     if (i >= n) return;
     const int ch = (int)(i % c);
@@ -798,17 +839,25 @@ __global__ void k_pool_fwd(const bf16_t *X, bf16_t *Y, Shape so, int c) {
     const int z = (int)(p % so.d);
     const int b = (int)(p / so.d);
     const int W = 2 * so.w, H = 2 * so.h, D = 2 * so.d;
-    float s = 0;
+    float s[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) s[j] = 0;
     for (int k = 0; k < 8; k++) {
         const int Z = 2 * z + (k >> 2), Y2 = 2 * y + ((k >> 1) & 1), X2 = 2 * x + (k & 1);
-        s += bf2f(X[((((long long)b * D + Z) * H + Y2) * W + X2) * c + ch]);
+        float v[V];
+        ld_bf<V>(X + ((((long long)b * D + Z) * H + Y2) * W + X2) * c + ch, v);
+#pragma unroll
+        for (int j = 0; j < V; j++) s[j] += v[j];
     }
-    Y[i] = f2bf(s * 0.125f);
+#pragma unroll
+    for (int j = 0; j < V; j++) s[j] *= 0.125f;
+    st_bf<V>(Y + i, s);
 }
 
 // average-pool backward: dX (fine) = dY (coarse) / 8
+template <int V>
 __global__ void k_pool_bwd(const bf16_t *dY, bf16_t *dX, Shape sf, int c) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * V;
     const long long n = (long long)sf.n * sf.d * sf.h * sf.w * c;
     if (i >= n) return;
     const int ch = (int)(i % c);
@@ -820,12 +869,17 @@ __global__ void k_pool_bwd(const bf16_t *dY, bf16_t *dX, Shape sf, int c) {
     const int z = (int)(p % sf.d);
     const int b = (int)(p / sf.d);
     const long long q = ((((long long)b * (sf.d / 2) + z / 2) * (sf.h / 2) + y / 2) * (sf.w / 2) + x / 2);
-    dX[i] = f2bf(bf2f(dY[q * c + ch]) * 0.125f);
+    float v[V];
+    ld_bf<V>(dY + q * c + ch, v);
+#pragma unroll
+    for (int j = 0; j < V; j++) v[j] *= 0.125f;
+    st_bf<V>(dX + i, v);
 }
 
 // nearest x2 upsampling: Y (fine) from X (coarse); optional skip added
+template <int V>
 __global__ void k_up_fwd(const bf16_t *X, const bf16_t *skip, bf16_t *Y, Shape sf, int c) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * V;
     const long long n = (long long)sf.n * sf.d * sf.h * sf.w * c;
     if (i >= n) return;
     const int ch = (int)(i % c);
@@ -837,14 +891,21 @@ __global__ void k_up_fwd(const bf16_t *X, const bf16_t *skip, bf16_t *Y, Shape s
     const int z = (int)(p % sf.d);
     const int b = (int)(p / sf.d);
     const long long q = ((((long long)b * (sf.d / 2) + z / 2) * (sf.h / 2) + y / 2) * (sf.w / 2) + x / 2);
-    float v = bf2f(X[q * c + ch]);
-    if (skip) v += bf2f(skip[i]);
-    Y[i] = f2bf(v);
+    float v[V];
+    ld_bf<V>(X + q * c + ch, v);
+    if (skip) {
+        float k[V];
+        ld_bf<V>(skip + i, k);
+#pragma unroll
+        for (int j = 0; j < V; j++) v[j] += k[j];
+    }
+    st_bf<V>(Y + i, v);
 }
 
 // upsampling backward: dX (coarse) = sum over the 8 children of dY (fine)
+template <int V>
 __global__ void k_up_bwd(const bf16_t *dY, bf16_t *dX, Shape so, int c) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * V;
     const long long n = (long long)so.n * so.d * so.h * so.w * c;
     if (i >= n) return;
     const int ch = (int)(i % c);
@@ -856,19 +917,30 @@ __global__ void k_up_bwd(const bf16_t *dY, bf16_t *dX, Shape so, int c) {
     const int z = (int)(p % so.d);
     const int b = (int)(p / so.d);
     const int W = 2 * so.w, H = 2 * so.h, D = 2 * so.d;
-    float s = 0;
+    float s[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) s[j] = 0;
     for (int k = 0; k < 8; k++) {
         const int Z = 2 * z + (k >> 2), Y2 = 2 * y + ((k >> 1) & 1), X2 = 2 * x + (k & 1);
-        s += bf2f(dY[((((long long)b * D + Z) * H + Y2) * W + X2) * c + ch]);
+        float v[V];
+        ld_bf<V>(dY + ((((long long)b * D + Z) * H + Y2) * W + X2) * c + ch, v);
+#pragma unroll
+        for (int j = 0; j < V; j++) s[j] += v[j];
     }
-    dX[i] = f2bf(s);
+    st_bf<V>(dX + i, s);
 }
 
 // a += b (gradient accumulation into a skip branch)
+template <int V>
 __global__ void k_add_inplace(bf16_t *a, const bf16_t *b, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * V;
     if (i >= n) return;
-    a[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+    float x[V], y[V];
+    ld_bf<V>(a + i, x);
+    ld_bf<V>(b + i, y);
+#pragma unroll
+    for (int j = 0; j < V; j++) x[j] += y[j];
+    st_bf<V>(a + i, x);
 }
 
 // MSE loss on FP32 predictions: dY = 2 (Y - T) / n (BF16); the loss is
@@ -1091,7 +1163,10 @@ int lbmnn_conv_wgrad(const void *dZ, const void *X, float *dW, float *db, int n,
 }
 
 int lbmnn_relu_bwd(void *dY, const void *Y, long long n) {
-    hipLaunchKernelGGL(k_relu_bwd, g1(n), dim3(256), 0, 0, (bf16_t *)dY, (const bf16_t *)Y, n);
+    if (n % 8 == 0)
+        hipLaunchKernelGGL(k_relu_bwd<8>, g1(n / 8), dim3(256), 0, 0, (bf16_t *)dY, (const bf16_t *)Y, n);
+    else
+        hipLaunchKernelGGL(k_relu_bwd<1>, g1(n), dim3(256), 0, 0, (bf16_t *)dY, (const bf16_t *)Y, n);
     return ws_check("relu backward");
 }
 
@@ -1104,33 +1179,50 @@ int lbmnn_bias_grad(const void *dZ, float *db, long long P, int c) {
 int lbmnn_pool_fwd(const void *X, void *Y, int n, int dOut, int hOut, int wOut, int c) {
     Shape so{n, dOut, hOut, wOut};
     const long long m = (long long)n * dOut * hOut * wOut * c;
-    hipLaunchKernelGGL(k_pool_fwd, g1(m), dim3(256), 0, 0, (const bf16_t *)X, (bf16_t *)Y, so, c);
+    if (c % 8 == 0)
+        hipLaunchKernelGGL(k_pool_fwd<8>, g1(m / 8), dim3(256), 0, 0, (const bf16_t *)X, (bf16_t *)Y, so, c);
+    else
+        hipLaunchKernelGGL(k_pool_fwd<1>, g1(m), dim3(256), 0, 0, (const bf16_t *)X, (bf16_t *)Y, so, c);
     return ws_check("pool forward");
 }
 
 int lbmnn_pool_bwd(const void *dY, void *dX, int n, int dIn, int hIn, int wIn, int c) {
     Shape sf{n, dIn, hIn, wIn};
     const long long m = (long long)n * dIn * hIn * wIn * c;
-    hipLaunchKernelGGL(k_pool_bwd, g1(m), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, sf, c);
+    if (c % 8 == 0)
+        hipLaunchKernelGGL(k_pool_bwd<8>, g1(m / 8), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, sf, c);
+    else
+        hipLaunchKernelGGL(k_pool_bwd<1>, g1(m), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, sf, c);
     return ws_check("pool backward");
 }
 
 int lbmnn_up_fwd(const void *X, const void *skip, void *Y, int n, int dOut, int hOut, int wOut, int c) {
     Shape sf{n, dOut, hOut, wOut};
     const long long m = (long long)n * dOut * hOut * wOut * c;
-    hipLaunchKernelGGL(k_up_fwd, g1(m), dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)skip, (bf16_t *)Y, sf, c);
+    if (c % 8 == 0)
+        hipLaunchKernelGGL(k_up_fwd<8>, g1(m / 8), dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)skip,
+                           (bf16_t *)Y, sf, c);
+    else
+        hipLaunchKernelGGL(k_up_fwd<1>, g1(m), dim3(256), 0, 0, (const bf16_t *)X, (const bf16_t *)skip, (bf16_t *)Y,
+                           sf, c);
     return ws_check("upsample forward");
 }
 
 int lbmnn_up_bwd(const void *dY, void *dX, int n, int dIn, int hIn, int wIn, int c) {
     Shape so{n, dIn, hIn, wIn};
     const long long m = (long long)n * dIn * hIn * wIn * c;
-    hipLaunchKernelGGL(k_up_bwd, g1(m), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, so, c);
+    if (c % 8 == 0)
+        hipLaunchKernelGGL(k_up_bwd<8>, g1(m / 8), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, so, c);
+    else
+        hipLaunchKernelGGL(k_up_bwd<1>, g1(m), dim3(256), 0, 0, (const bf16_t *)dY, (bf16_t *)dX, so, c);
     return ws_check("upsample backward");
 }
 
 int lbmnn_add_inplace(void *a, const void *b, long long n) {
-    hipLaunchKernelGGL(k_add_inplace, g1(n), dim3(256), 0, 0, (bf16_t *)a, (const bf16_t *)b, n);
+    if (n % 8 == 0)
+        hipLaunchKernelGGL(k_add_inplace<8>, g1(n / 8), dim3(256), 0, 0, (bf16_t *)a, (const bf16_t *)b, n);
+    else
+        hipLaunchKernelGGL(k_add_inplace<1>, g1(n), dim3(256), 0, 0, (bf16_t *)a, (const bf16_t *)b, n);
     return ws_check("add");
 }
 
