@@ -945,11 +945,30 @@ __global__ void k_add_inplace(bf16_t *a, const bf16_t *b, long long n) {
 
 // MSE loss on FP32 predictions: dY = 2 (Y - T) / n (BF16); the loss is
 // reduced per workgroup (wave shuffles, then LDS) into one atomic add.
-__global__ __launch_bounds__(256) void k_mse(const float *Yf, const float *T, bf16_t *dY, float *loss, long long n) {
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ Yf, const float *__restrict__ T,
+                                             bf16_t *__restrict__ dY, float *loss, long long n) {
     __shared__ float part[4];
     float l = 0.0f;
     const float inv = 1.0f / (float)n;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const long long stride = (long long)gridDim.x * 256;
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    // four grid strides at a time: their loads are issued before the first
+    // use; the loss still adds the elements of a lane in index order
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        float y[4], t[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            y[k] = Yf[i + k * stride];
+            t[k] = T[i + k * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float d = y[k] - t[k];
+            dY[i + k * stride] = f2bf(2.0f * d * inv);
+            l += d * d * inv;
+        }
+    }
+    for (; i < n; i += stride) {
         const float d = Yf[i] - T[i];
         dY[i] = f2bf(2.0f * d * inv);
         l += d * d * inv;
