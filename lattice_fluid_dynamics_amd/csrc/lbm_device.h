@@ -140,6 +140,11 @@ int lbmnn_synchronize(void);
 /* Zeroes n float buffers (n <= LBMNN_ZERO_MAX) with one asynchronous launch. */
 #define LBMNN_ZERO_MAX 40
 int lbmnn_zero_list(float *const *ptrs, const long long *counts, int n);
+/* AdamW step of count <= LBMNN_ADAM_MAX tensors in one launch, each element
+ * as lbmnn_adamw updates it (wb[k] may be NULL). */
+#define LBMNN_ADAM_MAX 32
+int lbmnn_adamw_list(float *const *w, void *const *wb, const float *const *g, float *const *m, float *const *v,
+                     const long long *n, const float *wd, int count, float lr, float b1, float b2, float eps, int t);
 /* Pinned host memory (hipHostMalloc) and an asynchronous device-to-host
  * copy ordered after the work already queued by the network. */
 void *lbmnn_host_alloc(size_t bytes);

@@ -333,11 +333,22 @@ float LBM_WarmStartTrainStep(LBMWarmStart *ws, const void *input, const float *t
     }
     int ok = queue_gradients(ws, input, target, batch);
     ws->step++;
+    /* every weight and bias tensor in one AdamW launch, then the BF16 copies */
+    float *aw[2 * NCONV], *am[2 * NCONV], *av[2 * NCONV], awd[2 * NCONV];
+    void *awb[2 * NCONV];
+    const float *ag[2 * NCONV];
+    long long an[2 * NCONV];
+    for (int k = 0; k < NCONV; k++) {
+        Conv *c = &ws->conv[k];
+        aw[2 * k] = c->w, awb[2 * k] = c->wb, ag[2 * k] = c->gw, am[2 * k] = c->mw, av[2 * k] = c->vw;
+        an[2 * k] = c->nw, awd[2 * k] = weightDecay;
+        aw[2 * k + 1] = c->b, awb[2 * k + 1] = NULL, ag[2 * k + 1] = c->gb, am[2 * k + 1] = c->mb;
+        av[2 * k + 1] = c->vb, an[2 * k + 1] = COUT[k], awd[2 * k + 1] = 0.0f;
+    }
+    ok = ok && lbmnn_adamw_list(aw, awb, ag, am, av, an, awd, 2 * NCONV, lr, 0.9f, 0.999f, 1e-8f, ws->step);
     for (int k = 0; k < NCONV && ok; k++) {
         Conv *c = &ws->conv[k];
-        ok = lbmnn_adamw(c->w, c->wb, c->gw, c->mw, c->vw, c->nw, lr, 0.9f, 0.999f, 1e-8f, weightDecay, ws->step) &&
-             lbmnn_adamw(c->b, NULL, c->gb, c->mb, c->vb, COUT[k], lr, 0.9f, 0.999f, 1e-8f, 0.0f, ws->step) &&
-             lbmnn_conv_pack(c->wb, c->wm, c->wp, c->wmp, CIN[k], COUT[k]);
+        ok = lbmnn_conv_pack(c->wb, c->wm, c->wp, c->wmp, CIN[k], COUT[k]);
     }
     if (!ok || !lbmnn_synchronize()) {
         ws_err("LBM_WarmStartTrainStep: device error");

@@ -995,6 +995,38 @@ __global__ void k_adamw(float *w, bf16_t *wb, const float *g, float *m, float *v
     if (wb) wb[i] = f2bf(wi);
 }
 
+// AdamW over up to LBMNN_ADAM_MAX tensors in one launch: element i of the
+// concatenation belongs to tensor k with start[k] <= i < start[k + 1]; each
+// element is updated exactly as k_adamw updates it.
+struct AdamList {
+    float *w[LBMNN_ADAM_MAX];
+    bf16_t *wb[LBMNN_ADAM_MAX];
+    const float *g[LBMNN_ADAM_MAX];
+    float *m[LBMNN_ADAM_MAX];
+    float *v[LBMNN_ADAM_MAX];
+    float wd[LBMNN_ADAM_MAX];
+    long long start[LBMNN_ADAM_MAX + 1];
+    int count;
+    float lr, b1, b2, eps, c1, c2;
+};
+
+__global__ void k_adamw_list(AdamList a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.start[a.count]) return;
+    int k = 0;
+    while (k + 1 < a.count && a.start[k + 1] <= i) k++;
+    const long long j = i - a.start[k];
+    const float gi = a.g[k][j];
+    const float mi = a.b1 * a.m[k][j] + (1.0f - a.b1) * gi;
+    const float vi = a.b2 * a.v[k][j] + (1.0f - a.b2) * gi * gi;
+    a.m[k][j] = mi;
+    a.v[k][j] = vi;
+    float wi = a.w[k][j] * (1.0f - a.lr * a.wd[k]);
+    wi -= a.lr * (mi / a.c1) / (sqrtf(vi / a.c2) + a.eps);
+    a.w[k][j] = wi;
+    if (a.wb[k]) a.wb[k][j] = f2bf(wi);
+}
+
 __global__ void k_f32_to_bf16(const float *a, bf16_t *b, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = f2bf(a[i]);
@@ -1255,6 +1287,31 @@ int lbmnn_adamw(float *w, void *wb, const float *g, float *m, float *v, long lon
                 float eps, float wd, int t) {
     const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
     hipLaunchKernelGGL(k_adamw, g1(n), dim3(256), 0, 0, w, (bf16_t *)wb, g, m, v, n, lr, b1, b2, eps, wd, c1, c2);
+    return ws_check("adamw");
+}
+
+int lbmnn_adamw_list(float *const *w, void *const *wb, const float *const *g, float *const *m, float *const *v,
+                     const long long *n, const float *wd, int count, float lr, float b1, float b2, float eps, int t) {
+    if (count <= 0 || count > LBMNN_ADAM_MAX) return 0;
+    AdamList a;
+    a.count = count;
+    a.start[0] = 0;
+    for (int k = 0; k < count; k++) {
+        a.w[k] = w[k];
+        a.wb[k] = (bf16_t *)wb[k];
+        a.g[k] = g[k];
+        a.m[k] = m[k];
+        a.v[k] = v[k];
+        a.wd[k] = wd[k];
+        a.start[k + 1] = a.start[k] + n[k];
+    }
+    a.lr = lr;
+    a.b1 = b1;
+    a.b2 = b2;
+    a.eps = eps;
+    a.c1 = 1.0f - powf(b1, (float)t);
+    a.c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adamw_list, g1(a.start[count]), dim3(256), 0, 0, a);
     return ws_check("adamw");
 }
 
