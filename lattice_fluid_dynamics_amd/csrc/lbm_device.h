@@ -143,6 +143,10 @@ int lbmnn_zero_list(float *const *ptrs, const long long *counts, int n);
 /* AdamW step of count <= LBMNN_ADAM_MAX tensors in one launch, each element
  * as lbmnn_adamw updates it (wb[k] may be NULL). */
 #define LBMNN_ADAM_MAX 32
+/* lbmnn_conv_pack of count <= LBMNN_PACK_MAX layers in one launch. */
+#define LBMNN_PACK_MAX 16
+int lbmnn_conv_pack_list(const void *const *W, void *const *Wm, void *const *Wp, void *const *Wmp, const int *cin,
+                         const int *cout, int count);
 int lbmnn_adamw_list(float *const *w, void *const *wb, const float *const *g, float *const *m, float *const *v,
                      const long long *n, const float *wd, int count, float lr, float b1, float b2, float eps, int t);
 /* Pinned host memory (hipHostMalloc) and an asynchronous device-to-host

@@ -113,6 +113,19 @@ void LBM_WarmStartFree(LBMWarmStart *ws) {
     free(ws);
 }
 
+/* mirrored and packed copies of every layer's BF16 weights, one launch */
+static int pack_all(LBMWarmStart *ws) {
+    const void *w[NCONV];
+    void *wm[NCONV], *wp[NCONV], *wmp[NCONV];
+    for (int k = 0; k < NCONV; k++) {
+        w[k] = ws->conv[k].wb;
+        wm[k] = ws->conv[k].wm;
+        wp[k] = ws->conv[k].wp;
+        wmp[k] = ws->conv[k].wmp;
+    }
+    return lbmnn_conv_pack_list(w, wm, wp, wmp, CIN, COUT, NCONV);
+}
+
 static int refresh_bf16(LBMWarmStart *ws, int k) {
     Conv *c = &ws->conv[k];
     return lbmnn_f32_to_bf16(c->w, c->wb, c->nw) && lbmnn_conv_pack(c->wb, c->wm, c->wp, c->wmp, CIN[k], COUT[k]);
@@ -346,10 +359,7 @@ float LBM_WarmStartTrainStep(LBMWarmStart *ws, const void *input, const float *t
         av[2 * k + 1] = c->vb, an[2 * k + 1] = COUT[k], awd[2 * k + 1] = 0.0f;
     }
     ok = ok && lbmnn_adamw_list(aw, awb, ag, am, av, an, awd, 2 * NCONV, lr, 0.9f, 0.999f, 1e-8f, ws->step);
-    for (int k = 0; k < NCONV && ok; k++) {
-        Conv *c = &ws->conv[k];
-        ok = lbmnn_conv_pack(c->wb, c->wm, c->wp, c->wmp, CIN[k], COUT[k]);
-    }
+    ok = ok && pack_all(ws);
     if (!ok || !lbmnn_synchronize()) {
         ws_err("LBM_WarmStartTrainStep: device error");
         return -1.0f;

@@ -341,10 +341,10 @@ uint4 *pack_buffer(size_t bytes) {
 // k_conv_mirror); and Wm packed (Wmp, a convolution with cin and cout
 // swapped).  Threads [0, n8) take one 8-channel group of W each, threads
 // [n8, 2 n8) one uint4 of Wmp.
-__global__ void k_conv_pack(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wm, uint4 *__restrict__ Wp,
-                            uint4 *__restrict__ Wmp, int cin, int cout) {
+__device__ __forceinline__ void pack_one(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wm,
+                                         uint4 *__restrict__ Wp, uint4 *__restrict__ Wmp, int cin, int cout,
+                                         long long i) {
     const long long n8 = (long long)cout * 27 * cin / 8;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n8) {
         const int ci = (int)(i % (cin / 8)) * 8;
         const int t = (int)((i / (cin / 8)) % 27);
@@ -370,6 +370,31 @@ __global__ void k_conv_pack(const bf16_t *__restrict__ W, bf16_t *__restrict__ W
 #pragma unroll
     for (int j = 0; j < 8; j++) h[j] = W[((long long)(c + j) * 27 + (26 - tm)) * cin + o];
     Wmp[i] = v;
+}
+
+__global__ void k_conv_pack(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wm, uint4 *__restrict__ Wp,
+                            uint4 *__restrict__ Wmp, int cin, int cout) {
+    pack_one(W, Wm, Wp, Wmp, cin, cout, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// k_conv_pack for up to LBMNN_PACK_MAX layers in one launch: thread i of the
+// concatenation works for layer k with start[k] <= i < start[k + 1].
+struct PackList {
+    const bf16_t *W[LBMNN_PACK_MAX];
+    bf16_t *Wm[LBMNN_PACK_MAX];
+    uint4 *Wp[LBMNN_PACK_MAX];
+    uint4 *Wmp[LBMNN_PACK_MAX];
+    int cin[LBMNN_PACK_MAX], cout[LBMNN_PACK_MAX];
+    long long start[LBMNN_PACK_MAX + 1];
+    int count;
+};
+
+__global__ void k_conv_pack_list(PackList a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.start[a.count]) return;
+    int k = 0;
+    while (k + 1 < a.count && a.start[k + 1] <= i) k++;
+    pack_one(a.W[k], a.Wm[k], a.Wp[k], a.Wmp[k], a.cin[k], a.cout[k], i - a.start[k]);
 }
 
 // Wp alone (the first half of k_conv_pack), for callers without packed copies.
@@ -1055,6 +1080,26 @@ int lbmnn_conv_pack(const void *W, void *Wm, void *Wp, void *Wmp, int cin, int c
     const long long n8 = (long long)cout * 27 * cin / 8;
     hipLaunchKernelGGL(k_conv_pack, g1(2 * n8), dim3(256), 0, 0, (const bf16_t *)W, (bf16_t *)Wm, (uint4 *)Wp,
                        (uint4 *)Wmp, cin, cout);
+    return ws_check("weight pack");
+}
+
+int lbmnn_conv_pack_list(const void *const *W, void *const *Wm, void *const *Wp, void *const *Wmp, const int *cin,
+                         const int *cout, int count) {
+    if (count <= 0 || count > LBMNN_PACK_MAX) return 0;
+    PackList a;
+    a.count = count;
+    a.start[0] = 0;
+    for (int k = 0; k < count; k++) {
+        if (cin[k] % 32 != 0 || cout[k] % 32 != 0) return 0;
+        a.W[k] = (const bf16_t *)W[k];
+        a.Wm[k] = (bf16_t *)Wm[k];
+        a.Wp[k] = (uint4 *)Wp[k];
+        a.Wmp[k] = (uint4 *)Wmp[k];
+        a.cin[k] = cin[k];
+        a.cout[k] = cout[k];
+        a.start[k + 1] = a.start[k] + 2LL * cout[k] * 27 * cin[k] / 8;
+    }
+    hipLaunchKernelGGL(k_conv_pack_list, g1(a.start[count]), dim3(256), 0, 0, a);
     return ws_check("weight pack");
 }
 
