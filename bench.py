@@ -185,7 +185,7 @@ def run(args):
         parts = lbm.DeviceParticles(w["particles"])
         lbm.LBM_SeedParticles(g, parts, 0)
 
-    tuned_block, tune_ms = lbm.LBM_Autotune(g, 10, w["u"], 0.0, 0.0)
+    tuned_block, tune_ms = lbm.LBM_Autotune(g, 20, w["u"], 0.0, 0.0)
     _, streaming = lbm.LBM_GetKernelConfig(g)
     log(f"autotune: workgroup {tuned_block}, streaming mode {streaming} "
         f"(best ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
