@@ -336,68 +336,55 @@ uint4 *pack_buffer(size_t bytes) {
 // and MFMA column block nb, the 64 uint4 read by the lanes l = 16 kq + r of
 // a wave, lane l holding W[32 cb + 16 nb + r][t][32 ch + 8 kq .. + 8]:
 //   Wp[(((cb * cin / 32 + ch) * 27 + t) * 2 + nb) * 64 + l].
-// k_conv_pack writes, from W[co][t][ci] in one launch: the packed forward
-// weights Wp; the input-gradient weights Wm[ci][26 - t][co] (as
-// k_conv_mirror); and Wm packed (Wmp, a convolution with cin and cout
-// swapped).  Threads [0, n8) take one 8-channel group of W each, threads
-// [n8, 2 n8) one uint4 of Wmp.
-__device__ __forceinline__ void pack_one(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wm,
-                                         uint4 *__restrict__ Wp, uint4 *__restrict__ Wmp, int cin, int cout,
-                                         long long i) {
-    const long long n8 = (long long)cout * 27 * cin / 8;
-    if (i < n8) {
-        const int ci = (int)(i % (cin / 8)) * 8;
-        const int t = (int)((i / (cin / 8)) % 27);
-        const int co = (int)(i / ((long long)(cin / 8) * 27));
-        const uint4 v = *(const uint4 *)(W + ((long long)co * 27 + t) * cin + ci);
-        const int lane = 16 * ((ci % 32) / 8) + co % 16;
-        Wp[(((long long)(co / 32) * (cin / 32) + ci / 32) * 27 + t) * 128 + ((co % 32) / 16) * 64 + lane] = v;
-        const bf16_t *h = (const bf16_t *)&v;
-#pragma unroll
-        for (int j = 0; j < 8; j++) Wm[((long long)(ci + j) * 27 + (26 - t)) * cout + co] = h[j];
-        return;
-    }
-    i -= n8;
-    if (i >= n8) return;
-    // Wmp of the swapped convolution: output channel o (an input channel of
-    // W), tap tm, input channels c .. c + 8 (output channels of W)
-    const int lane = (int)(i % 64), nb = (int)((i / 64) % 2), tm = (int)((i / 128) % 27);
-    const long long blk = i / (128 * 27);  // = ob * (cout / 32) + ch
-    const int ch = (int)(blk % (cout / 32)), ob = (int)(blk / (cout / 32));
-    const int o = 32 * ob + 16 * nb + (lane % 16), c = 32 * ch + 8 * (lane / 16);
-    uint4 v;
-    bf16_t *h = (bf16_t *)&v;
-#pragma unroll
-    for (int j = 0; j < 8; j++) h[j] = W[((long long)(c + j) * 27 + (26 - tm)) * cin + o];
-    Wmp[i] = v;
-}
-
-__global__ void k_conv_pack(const bf16_t *__restrict__ W, bf16_t *__restrict__ Wm, uint4 *__restrict__ Wp,
-                            uint4 *__restrict__ Wmp, int cin, int cout) {
-    pack_one(W, Wm, Wp, Wmp, cin, cout, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// k_conv_pack for up to LBMNN_PACK_MAX layers in one launch: thread i of the
-// concatenation works for layer k with start[k] <= i < start[k + 1].
+// k_conv_pack writes, from W[co][t][ci]: the packed forward weights Wp; the
+// input-gradient weights Wm[ci][26 - t][co] (as k_conv_mirror); and Wm
+// packed (Wmp, the same layout for the convolution with cin and cout
+// swapped).  One workgroup of 128 lanes per (layer, tap, 32 x 32 channel
+// tile), up to LBMNN_PACK_MAX layers per launch: lane l reads 8 input
+// channels of output channel l / 4 (one 16-byte load) and writes them to Wp;
+// the tile is transposed through LDS, and lane l then writes 8 consecutive
+// output channels of input channel l / 4 to Wm and Wmp, 16 bytes each.
 struct PackList {
     const bf16_t *W[LBMNN_PACK_MAX];
     bf16_t *Wm[LBMNN_PACK_MAX];
     uint4 *Wp[LBMNN_PACK_MAX];
     uint4 *Wmp[LBMNN_PACK_MAX];
     int cin[LBMNN_PACK_MAX], cout[LBMNN_PACK_MAX];
-    long long start[LBMNN_PACK_MAX + 1];
+    long long tstart[LBMNN_PACK_MAX + 1];  // first tile of each layer
     int count;
 };
 
-__global__ void k_conv_pack_list(PackList a) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.start[a.count]) return;
+__global__ __launch_bounds__(128) void k_conv_pack(PackList a) {
+    __shared__ bf16_t tile[32][40];  // [co][ci] of the tile, rows padded
+    const long long tb = blockIdx.x;
     int k = 0;
-    while (k + 1 < a.count && a.start[k + 1] <= i) k++;
-    pack_one(a.W[k], a.Wm[k], a.Wp[k], a.Wmp[k], a.cin[k], a.cout[k], i - a.start[k]);
+    while (k + 1 < a.count && a.tstart[k + 1] <= tb) k++;
+    long long rest = tb - a.tstart[k];
+    const int cin = a.cin[k], cout = a.cout[k];
+    const int cib = (int)(rest % (cin / 32));
+    rest /= cin / 32;
+    const int cob = (int)(rest % (cout / 32));
+    const int t = (int)(rest / (cout / 32));
+    const int tid = threadIdx.x;
+    const int co = cob * 32 + tid / 4, ci = cib * 32 + 8 * (tid % 4);
+    const uint4 v = *(const uint4 *)(a.W[k] + ((long long)co * 27 + t) * cin + ci);
+    a.Wp[k][(((long long)cob * (cin / 32) + cib) * 27 + t) * 128 + ((co % 32) / 16) * 64 + 16 * ((ci % 32) / 8) +
+            co % 16] = v;
+    *(uint4 *)&tile[tid / 4][8 * (tid % 4)] = v;
+    __syncthreads();
+    // o: an input channel of W (output channel of the mirrored convolution),
+    // c .. c + 8: output channels of W (its input channels), tap 26 - t
+    const int o = cib * 32 + tid / 4, c = cob * 32 + 8 * (tid % 4), tm = 26 - t;
+    uint4 u;
+    bf16_t *h = (bf16_t *)&u;
+#pragma unroll
+    for (int j = 0; j < 8; j++) h[j] = tile[8 * (tid % 4) + j][tid / 4];
+    *(uint4 *)(a.Wm[k] + ((long long)o * 27 + tm) * cout + c) = u;
+    a.Wmp[k][(((long long)cib * (cout / 32) + cob) * 27 + tm) * 128 + ((o % 32) / 16) * 64 + 16 * ((c % 32) / 8) +
+             o % 16] = u;
 }
 
-// Wp alone (the first half of k_conv_pack), for callers without packed copies.
+// Wp alone, for callers without packed copies (one lane per 16-byte group of W).
 __global__ void k_conv_pack_fwd(const bf16_t *__restrict__ W, uint4 *__restrict__ Wp, int cin, int cout) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)cout * 27 * cin / 8) return;
@@ -1075,20 +1062,12 @@ static int ws_check(const char *what) {
     return 1;
 }
 
-int lbmnn_conv_pack(const void *W, void *Wm, void *Wp, void *Wmp, int cin, int cout) {
-    if (cin % 32 != 0 || cout % 32 != 0) return 0;
-    const long long n8 = (long long)cout * 27 * cin / 8;
-    hipLaunchKernelGGL(k_conv_pack, g1(2 * n8), dim3(256), 0, 0, (const bf16_t *)W, (bf16_t *)Wm, (uint4 *)Wp,
-                       (uint4 *)Wmp, cin, cout);
-    return ws_check("weight pack");
-}
-
 int lbmnn_conv_pack_list(const void *const *W, void *const *Wm, void *const *Wp, void *const *Wmp, const int *cin,
                          const int *cout, int count) {
     if (count <= 0 || count > LBMNN_PACK_MAX) return 0;
     PackList a;
     a.count = count;
-    a.start[0] = 0;
+    a.tstart[0] = 0;
     for (int k = 0; k < count; k++) {
         if (cin[k] % 32 != 0 || cout[k] % 32 != 0) return 0;
         a.W[k] = (const bf16_t *)W[k];
@@ -1097,10 +1076,14 @@ int lbmnn_conv_pack_list(const void *const *W, void *const *Wm, void *const *Wp,
         a.Wmp[k] = (uint4 *)Wmp[k];
         a.cin[k] = cin[k];
         a.cout[k] = cout[k];
-        a.start[k + 1] = a.start[k] + 2LL * cout[k] * 27 * cin[k] / 8;
+        a.tstart[k + 1] = a.tstart[k] + 27LL * (cout[k] / 32) * (cin[k] / 32);
     }
-    hipLaunchKernelGGL(k_conv_pack_list, g1(a.start[count]), dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(k_conv_pack, dim3((unsigned)a.tstart[count]), dim3(128), 0, 0, a);
     return ws_check("weight pack");
+}
+
+int lbmnn_conv_pack(const void *W, void *Wm, void *Wp, void *Wmp, int cin, int cout) {
+    return lbmnn_conv_pack_list(&W, &Wm, &Wp, &Wmp, &cin, &cout, 1);
 }
 
 // cin % 32 == 0, cout % 16 == 0.  Wp: the packed weights of k_conv_pack
