@@ -45,6 +45,7 @@ def config2_grid(storage: int):
     lbm.LBM_SetSolidMesh(g, tris, bbox)
     area = lbm.LBM_ComputeProjectedArea(g, 0)
     assert lbm.LBM_InitializeFlow(g, 0.06, 0, 0) == 1
+    lbm.LBM_Autotune(g, 20, 0.06)  # workgroup and streaming mode; the state is restored
     return g, area, len(tris)
 
 
