@@ -235,7 +235,10 @@ int LBM_Autotune(LBMGrid *grid, int steps, float inletVelX, float inletVelY, flo
                  float *msPerStep);
 
 /* Sets the step kernel's workgroup size (128, 256 or 512) instead of
- * tuning it; the results do not depend on it.  1 on success. */
+ * tuning it; the results do not depend on it.  1 on success.  This and the
+ * streaming mode below belong to the device state of the current storage
+ * format: an LBM_InitializeFlow that switches populationStorage starts again
+ * from the defaults (256 lanes, mode 0), so set them after it. */
 int LBM_SetBlockSize(LBMGrid *grid, int blockSize);
 
 /* Selects the cache policy of the step kernel's population loads and stores
