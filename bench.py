@@ -186,8 +186,8 @@ def run(args):
         lbm.LBM_SeedParticles(g, parts, 0)
 
     tuned_block, tune_ms = lbm.LBM_Autotune(g, 20, w["u"], 0.0, 0.0)
-    _, streaming = lbm.LBM_GetKernelConfig(g)
-    log(f"autotune: workgroup {tuned_block}, streaming mode {streaming} "
+    _, streaming, xcd_order = lbm.LBM_GetKernelConfig(g)
+    log(f"autotune: workgroup {tuned_block}, streaming mode {streaming}, XCD-contiguous order {xcd_order} "
         f"(best ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
     if args.warmup:
         run_frames(lbm, g, parts, w, args.warmup, 0)
@@ -211,7 +211,7 @@ def run(args):
                 "kernel": "k_step (fused stream+collide)", "bytes_per_cell": bpc,
                 "avg_launch_ms": round(main_ms / timed_steps, 5), "timed_launches": timed_steps,
                 "copy_kernel_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
-                "workgroup": tuned_block, "streaming_mode": streaming,
+                "workgroup": tuned_block, "streaming_mode": streaming, "xcd_contiguous": xcd_order,
                 "boundary_prepass_avg_ms": round(bnd_ms / max(timed_steps, 1), 5)}
     cpu = None
     if args.cpu_steps > 0:

@@ -226,11 +226,12 @@ int LBM_WriteVTI(LBMGrid *grid, const char *dir, int step);
 
 /* Times `steps` steps of the grid for each candidate configuration of the
  * step kernel: workgroup size 128, 256 or 512 lanes, each with streaming
- * mode 0, 1 and 3 (see LBM_SetStreamingAccess; FP64 storage: mode 0 only).
- * Uses the given inlet velocity, restores the simulation state after each
- * trial, keeps the fastest configuration and returns its workgroup size
- * (0 on failure).  msPerStep (3 floats, may be NULL) receives the best time
- * per workgroup size. */
+ * mode 0, 1 and 3 (see LBM_SetStreamingAccess; FP64 storage: mode 0 only)
+ * and each in both workgroup orders (see LBM_SetBlockOrder).  Uses the given
+ * inlet velocity, restores the simulation state after each trial, keeps the
+ * fastest configuration and returns its workgroup size (0 on failure).
+ * msPerStep (3 floats, may be NULL) receives the best time per workgroup
+ * size. */
 int LBM_Autotune(LBMGrid *grid, int steps, float inletVelX, float inletVelY, float inletVelZ,
                  float *msPerStep);
 
@@ -248,9 +249,15 @@ int LBM_SetBlockSize(LBMGrid *grid, int blockSize);
  * 0 for a mode outside 0-3. */
 int LBM_SetStreamingAccess(LBMGrid *grid, int mode);
 
-/* The step kernel's current workgroup size and streaming mode (either
- * pointer may be NULL).  1 on success. */
-int LBM_GetKernelConfig(const LBMGrid *grid, int *blockSize, int *streamingMode);
+/* Workgroup order of the step kernel instead of tuning it: 1 gives each of
+ * the 8 XCDs one contiguous range of cells, 0 lets consecutive workgroups
+ * go to different XCDs (the dispatch order).  The results do not depend on
+ * it.  1 on success. */
+int LBM_SetBlockOrder(LBMGrid *grid, int xcdContiguous);
+
+/* The step kernel's current workgroup size, streaming mode and workgroup
+ * order (any pointer may be NULL).  1 on success. */
+int LBM_GetKernelConfig(const LBMGrid *grid, int *blockSize, int *streamingMode, int *xcdContiguous);
 
 /* Attainable HBM bandwidth in GB/s: read + write bytes of a plain device copy
  * of `bytes` bytes over its mean launch time (iters launches). */
@@ -285,11 +292,12 @@ void LBM_BatchStep(LBMBatch *batch, int n);
 int LBM_BatchProfile(LBMBatch *batch, int n, int useGraph, float *ms);
 int LBM_BatchAutotune(LBMBatch *batch, int steps, float *msPerStep);
 int LBM_BatchSetBlockSize(LBMBatch *batch, int blockSize);
-/* As LBM_SetStreamingAccess and LBM_GetKernelConfig, for the batch step
- * kernel (LBM_BatchAutotune tunes the streaming mode too).  Available after
- * LBM_BatchInitialize. */
+/* As LBM_SetStreamingAccess, LBM_SetBlockOrder and LBM_GetKernelConfig, for
+ * the batch step kernel (LBM_BatchAutotune tunes all three).  Available
+ * after LBM_BatchInitialize. */
 int LBM_BatchSetStreamingAccess(LBMBatch *batch, int mode);
-int LBM_BatchGetKernelConfig(const LBMBatch *batch, int *blockSize, int *streamingMode);
+int LBM_BatchSetBlockOrder(LBMBatch *batch, int xcdContiguous);
+int LBM_BatchGetKernelConfig(const LBMBatch *batch, int *blockSize, int *streamingMode, int *xcdContiguous);
 int LBM_BatchTunnelCount(const LBMBatch *batch);
 long long LBM_BatchTotalCells(const LBMBatch *batch);
 /* out: 6 doubles per tunnel (F xyz, pressure part xyz, FP64 sums); fixedOut

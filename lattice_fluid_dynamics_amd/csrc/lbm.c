@@ -838,11 +838,18 @@ int LBM_SetStreamingAccess(LBMGrid *g, int mode) {
     return 1;
 }
 
-int LBM_GetKernelConfig(const LBMGrid *g, int *blockSize, int *streaming) {
+int LBM_SetBlockOrder(LBMGrid *g, int xcdContiguous) {
+    if (!g || !impl_of(g)->dev) return 0;
+    lbmdev_set_block_order(impl_of(g)->dev, xcdContiguous);
+    return 1;
+}
+
+int LBM_GetKernelConfig(const LBMGrid *g, int *blockSize, int *streaming, int *xcdContiguous) {
     if (!g || !impl_of((LBMGrid *)g)->dev) return 0;
     const LbmDev *d = impl_of((LBMGrid *)g)->dev;
     if (blockSize) *blockSize = lbmdev_block_size(d);
     if (streaming) *streaming = lbmdev_streaming(d);
+    if (xcdContiguous) *xcdContiguous = lbmdev_block_order(d);
     return 1;
 }
 

@@ -240,10 +240,17 @@ int LBM_BatchSetStreamingAccess(LBMBatch *b, int mode) {
     return b && b->dev ? lbmbatch_set_streaming(b->dev, mode) : 0;
 }
 
-int LBM_BatchGetKernelConfig(const LBMBatch *b, int *blockSize, int *streamingMode) {
+int LBM_BatchSetBlockOrder(LBMBatch *b, int xcdContiguous) {
+    if (!b || !b->dev) return 0;
+    lbmbatch_set_block_order(b->dev, xcdContiguous);
+    return 1;
+}
+
+int LBM_BatchGetKernelConfig(const LBMBatch *b, int *blockSize, int *streamingMode, int *xcdContiguous) {
     if (!b || !b->dev) return 0;
     if (blockSize) *blockSize = lbmbatch_block_size(b->dev);
     if (streamingMode) *streamingMode = lbmbatch_streaming(b->dev);
+    if (xcdContiguous) *xcdContiguous = lbmbatch_block_order(b->dev);
     return 1;
 }
 

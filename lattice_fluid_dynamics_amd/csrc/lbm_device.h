@@ -88,6 +88,8 @@ int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, float *ms_per_
 int lbmdev_block_size(const LbmDev *d);
 int lbmdev_set_streaming(LbmDev *d, int mode);
 int lbmdev_streaming(const LbmDev *d);
+void lbmdev_set_block_order(LbmDev *d, int xcd_contiguous);
+int lbmdev_block_order(const LbmDev *d);
 int lbmdev_export_populations_f64(LbmDev *d, int periodicYZ, double *f_post_stream, double *f_post_collision);
 void *lbmdev_stream(LbmDev *d);
 
@@ -115,6 +117,8 @@ int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int steps, float 
 int lbmbatch_block_size(const LbmBatchDev *b);
 int lbmbatch_set_streaming(LbmBatchDev *b, int mode);
 int lbmbatch_streaming(const LbmBatchDev *b);
+void lbmbatch_set_block_order(LbmBatchDev *b, int xcd_contiguous);
+int lbmbatch_block_order(const LbmBatchDev *b);
 int lbmbatch_force(LbmBatchDev *b, int periodic, double *exact, long long *fixed);
 int lbmbatch_export(LbmBatchDev *b, int t, int periodic, double *f_post_stream, double *f_post_collision);
 int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host);

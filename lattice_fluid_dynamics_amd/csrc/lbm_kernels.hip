@@ -130,9 +130,13 @@ __device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, int s) {
     return m == 0 ? n : (__umulhi(n, m) >> s);
 }
 
-// Each of the 8 XCDs receives workgroups b, b+8, b+16, ... in dispatch order;
-// this gives every XCD one contiguous range of cells so neighbouring rows and
-// partially shared cache lines stay in one L2.
+// Each of the 8 XCDs receives workgroups b, b+8, b+16, ... in dispatch order.
+// xcd_remap gives every XCD one contiguous range of cells, so neighbouring
+// rows and partially shared cache lines stay in one L2; without it the eight
+// XCDs sweep the lattice side by side.  Lattices that partly fit the caches
+// gain up to 2 % from the remap; lattices far larger lose 6-9 % (512x256x256
+// FP32 and FP16), so the order is a launch parameter that the autotuner
+// measures (`remap` in StepArgs / BatchArgs).
 __device__ __forceinline__ int xcd_remap(int b, int n) {
     int q = n >> 3, r = n & 7, k = b & 7, j = b >> 3;
     return k * q + (k < r ? k : r) + j;
@@ -516,6 +520,7 @@ struct StepArgs {
     int use_ovr;
     float4 *vel;
     int write_vel;
+    int remap;  // 1: XCD-contiguous workgroup order (xcd_remap)
     Geo g;
     CollideParams cp;
 };
@@ -648,7 +653,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
 // NT = 1: non-temporal population loads and stores.
 template <int FMT, int OP, int PHASE, int BS, int ROW = 0, int NT = 0>
 __global__ __launch_bounds__(BS) void k_step(StepArgs<FMT> a) {
-    const int cell = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    const int cell = (a.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x) * BS + threadIdx.x;
     if (!ROW && cell >= a.g.ncell) return;
     step_cell<FMT, OP, PHASE, ROW, NT>(a, cell);
 }
@@ -678,6 +683,7 @@ struct BatchArgs {
     int write_vel;
     float cs2x18;
     int smag;
+    int remap;  // 1: XCD-contiguous workgroup order (xcd_remap)
     const TunnelDesc *desc;
     const int32_t *block_tunnel;
 };
@@ -686,7 +692,7 @@ struct BatchArgs {
 // NT = 1: non-temporal population loads and stores.
 template <int FMT, int OP, int PHASE, int BS, int ROW = 0, int NT = 0>
 __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int blk = b.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const TunnelDesc d = b.desc[b.block_tunnel[blk]];
     const int cell = (blk - d.blockStart) * BS + threadIdx.x;
     if (!ROW && cell >= d.g.ncell) return;
@@ -1184,6 +1190,7 @@ struct LbmDev {
     int nparts;  // partial slots allocated
     int bs;      // workgroup size of the step kernel
     int nt;      // LBMDEV_NT_A / LBMDEV_NT_B: non-temporal population access per phase
+    int remap;   // XCD-contiguous workgroup order in the step kernel
 };
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
@@ -1224,6 +1231,7 @@ extern "C" LbmDev *lbmdev_create(int nx, int ny, int nz, int storage, char *err,
     d->stride = (nc + 63) / 64 * 64;
     d->state = -1;
     d->bs = 256;
+    d->remap = 1;
     hipError_t e;
     const size_t popBytes = (size_t)19 * d->stride * d->elem;
     d->nparts = (int)((nc + BLOCK - 1) / BLOCK);
@@ -1465,6 +1473,7 @@ static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ov
     a.cp.iny = p->inletY;
     a.cp.inz = p->inletZ;
     a.cp.smag = p->useSmagorinsky == 1;
+    a.remap = d->remap;
     // FP64 storage uses global addressing and has no streaming variant
     if (FMT != LBMDEV_FP64 && (d->nt & (phase == 0 ? LBMDEV_NT_A : LBMDEV_NT_B)))
         launch_step_bs<FMT, OP, FMT != LBMDEV_FP64>(d->bs, phase, d->ncell, d->stream, a);
@@ -1836,6 +1845,7 @@ struct LbmBatchDev {
     int32_t *block_tunnel;
     int nblocks, bs;
     int smode;  // LBMDEV_NT_A / LBMDEV_NT_B streaming mode of the step kernel
+    int remap;  // XCD-contiguous workgroup order in the step kernel
     int state;
     hipStream_t stream;
     double *pd;
@@ -1846,7 +1856,7 @@ struct LbmBatchDev {
     hipGraphExec_t graph[2];
     int graph_ok;
     LbmBatchParams graph_params;
-    int graph_bs, graph_smode;
+    int graph_bs, graph_smode, graph_remap;
 };
 
 extern "C" void lbmbatch_destroy(LbmBatchDev *b);
@@ -1895,6 +1905,7 @@ extern "C" LbmBatchDev *lbmbatch_create(int storage, int nt, const int *dims, co
     b->elem = elem_size(storage);
     b->telem = storage == LBMDEV_FP64 ? 8 : 4;
     b->nt = nt;
+    b->remap = 1;
     b->state = -1;
     b->desc_h = (TunnelDesc *)calloc((size_t)nt, sizeof(TunnelDesc));
     b->dims = (int *)malloc(sizeof(int) * 3 * (size_t)nt);
@@ -2118,6 +2129,7 @@ static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase
     a.smag = p->useSmagorinsky == 1;
     a.desc = b->desc_d;
     a.block_tunnel = b->block_tunnel;
+    a.remap = b->remap;
     // row-aligned phase B when every tunnel's rows fill whole workgroups
     bool row = FMT != LBMDEV_FP64;
     for (int t = 0; t < b->nt && row; t++) {
@@ -2222,6 +2234,7 @@ static int batch_capture(LbmBatchDev *b, const LbmBatchParams *p) {
     b->graph_params = *p;
     b->graph_bs = b->bs;
     b->graph_smode = b->smode;
+    b->graph_remap = b->remap;
     return 1;
 }
 
@@ -2236,7 +2249,7 @@ extern "C" int lbmbatch_step(LbmBatchDev *b, const LbmBatchParams *p, int nsteps
     }
     if (use_graph && nsteps >= 2) {
         if (!b->graph_ok || !same_params(&b->graph_params, p) || b->graph_bs != b->bs ||
-            b->graph_smode != b->smode)
+            b->graph_smode != b->smode || b->graph_remap != b->remap)
             if (!batch_capture(b, p)) return 0;
         const int pairs = nsteps / 2, rem = nsteps % 2;
         for (int k = 0; k < pairs; k++) {
@@ -2266,6 +2279,10 @@ extern "C" int lbmbatch_set_streaming(LbmBatchDev *b, int mode) {
 }
 
 extern "C" int lbmbatch_streaming(const LbmBatchDev *b) { return b->smode; }
+
+extern "C" void lbmbatch_set_block_order(LbmBatchDev *b, int xcd_contiguous) { b->remap = xcd_contiguous != 0; }
+
+extern "C" int lbmbatch_block_order(const LbmBatchDev *b) { return b->remap; }
 
 extern "C" int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph, float *ms) {
     hipEvent_t t0, t1;
@@ -2425,6 +2442,10 @@ extern "C" int lbmdev_set_streaming(LbmDev *d, int mode) {
 
 extern "C" int lbmdev_streaming(const LbmDev *d) { return d->nt; }
 
+extern "C" void lbmdev_set_block_order(LbmDev *d, int xcd_contiguous) { d->remap = xcd_contiguous != 0; }
+
+extern "C" int lbmdev_block_order(const LbmDev *d) { return d->remap; }
+
 // ---------------------------------------------------------------- autotuner
 // Candidates are timed on the real lattice; the stored state (populations,
 // velocity, layout) is copied aside first and put back after every trial.
@@ -2461,15 +2482,17 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
     Snapshot s;
     const size_t pb = (size_t)19 * d->stride * d->elem, vb = (size_t)d->ncell * 16;
     if (!snapshot_take(s, d->pop, pb, d->vel, vb, d->stream)) return 0;
-    const int state0 = d->state, bs0 = d->bs, nt0 = d->nt;
-    int best = 0, bestNt = 0;
+    const int state0 = d->state, bs0 = d->bs, nt0 = d->nt, remap0 = d->remap;
+    int best = 0, bestNt = 0, bestRemap = 1;
     float bestMs = 0;
     int ok = 1;
-    // every workgroup size with each streaming mode (FP64: default access only)
-    const int ncand = d->fmt == LBMDEV_FP64 ? 3 : 9;
-    for (int c = 0; c < ncand && ok; c++) {
+    // every workgroup size with each streaming mode (FP64: default access
+    // only), in XCD-contiguous and in plain workgroup order
+    const int nmode = d->fmt == LBMDEV_FP64 ? 1 : 3;
+    for (int c = 0; c < 6 * nmode && ok; c++) {
         d->bs = kBlockSizes[c % 3];
-        d->nt = kStreamModes[c / 3];
+        d->nt = kStreamModes[(c / 3) % nmode];
+        d->remap = c < 3 * nmode;
         float tot = 0;
         ok = dispatch_steps(d, p, 2, NULL) &&  // warm-up
              lbmdev_profile_steps(d, p, steps, &tot, NULL, NULL);
@@ -2478,6 +2501,7 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
         if (ok && (best == 0 || ms < bestMs)) {
             best = d->bs;
             bestNt = d->nt;
+            bestRemap = d->remap;
             bestMs = ms;
         }
         ok = ok && snapshot_put(s, d->pop, d->vel, d->stream);
@@ -2487,6 +2511,7 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
     hipFree(s.vel);
     d->bs = ok ? best : bs0;
     d->nt = ok ? bestNt : nt0;
+    d->remap = ok ? bestRemap : remap0;
     return ok ? best : 0;
 }
 
@@ -2495,15 +2520,17 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
     Snapshot s;
     const size_t pb = (size_t)b->total_pop * b->elem, vb = (size_t)b->total_cells * 16;
     if (!snapshot_take(s, b->pop, pb, b->vel, vb, b->stream)) return 0;
-    const int state0 = b->state, bs0 = b->bs, smode0 = b->smode;
-    int best = 0, bestMode = 0;
+    const int state0 = b->state, bs0 = b->bs, smode0 = b->smode, remap0 = b->remap;
+    int best = 0, bestMode = 0, bestRemap = 1;
     float bestMs = 0;
     int ok = 1;
-    // every workgroup size with each streaming mode (FP64: default access only)
-    const int ncand = b->fmt == LBMDEV_FP64 ? 3 : 9;
-    for (int c = 0; c < ncand && ok; c++) {
+    // every workgroup size with each streaming mode (FP64: default access
+    // only), in XCD-contiguous and in plain workgroup order
+    const int nmode = b->fmt == LBMDEV_FP64 ? 1 : 3;
+    for (int c = 0; c < 6 * nmode && ok; c++) {
         ok = lbmbatch_set_block_size(b, kBlockSizes[c % 3]);
-        b->smode = kStreamModes[c / 3];
+        b->smode = kStreamModes[(c / 3) % nmode];
+        b->remap = c < 3 * nmode;
         float ms = 0;
         ok = ok && lbmbatch_step(b, p, 2, 1) && lbmbatch_profile(b, p, steps, 1, &ms);
         ms /= steps;
@@ -2511,6 +2538,7 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
         if (ok && (best == 0 || ms < bestMs)) {
             best = kBlockSizes[c % 3];
             bestMode = b->smode;
+            bestRemap = b->remap;
             bestMs = ms;
         }
         ok = ok && snapshot_put(s, b->pop, b->vel, b->stream);
@@ -2519,6 +2547,7 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
     hipFree(s.pop);
     hipFree(s.vel);
     b->smode = ok ? bestMode : smode0;
+    b->remap = ok ? bestRemap : remap0;
     if (!lbmbatch_set_block_size(b, ok ? best : bs0)) return 0;
     return ok ? best : 0;
 }

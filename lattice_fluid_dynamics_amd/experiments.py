@@ -104,6 +104,7 @@ def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 
     _emit({"experiment": "config4-sweep", "tunnels": ntun, "total_cells": cells, "steps": steps,
            "storage": {0: "fp32", 1: "fp16", 2: "bf16"}[storage], "setup_s": round(setup_s, 1),
            "autotune_block": best, "autotune_streaming_mode": b.kernel_config()[1],
+           "autotune_xcd_contiguous": b.kernel_config()[2],
            "autotune_ms_per_step": [round(float(v), 4) for v in ms],
            "ms_per_step_eager": round(ms_nograph, 4), "ms_per_step_graph": round(ms_graph, 4),
            "total_s": round(t / 1e3, 3), "aggregate_mlups": round(cells * steps / (t / 1e3) / 1e6, 1)})

@@ -99,7 +99,8 @@ SIGNATURES = {
     "LBM_Autotune": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p]),
     "LBM_SetBlockSize": (c_int, [GridP, c_int]),
     "LBM_SetStreamingAccess": (c_int, [GridP, c_int]),
-    "LBM_GetKernelConfig": (c_int, [GridP, c_int_p, c_int_p]),
+    "LBM_SetBlockOrder": (c_int, [GridP, c_int]),
+    "LBM_GetKernelConfig": (c_int, [GridP, c_int_p, c_int_p, c_int_p]),
     "LBM_MeasureCopyBandwidth": (c_float, [ctypes.c_ulonglong, c_int]),
     "LBM_BatchCreate": (c_void_p, [c_int, c_int, c_int, c_int, c_float, c_int]),
     "LBM_BatchFree": (None, [c_void_p]),
@@ -110,7 +111,8 @@ SIGNATURES = {
     "LBM_BatchAutotune": (c_int, [c_void_p, c_int, c_float_p]),
     "LBM_BatchSetBlockSize": (c_int, [c_void_p, c_int]),
     "LBM_BatchSetStreamingAccess": (c_int, [c_void_p, c_int]),
-    "LBM_BatchGetKernelConfig": (c_int, [c_void_p, c_int_p, c_int_p]),
+    "LBM_BatchSetBlockOrder": (c_int, [c_void_p, c_int]),
+    "LBM_BatchGetKernelConfig": (c_int, [c_void_p, c_int_p, c_int_p, c_int_p]),
     "LBM_BatchTunnelCount": (c_int, [c_void_p]),
     "LBM_BatchTotalCells": (ctypes.c_longlong, [c_void_p]),
     "LBM_BatchComputeForcesF64": (c_int, [c_void_p, c_double_p, ctypes.POINTER(ctypes.c_longlong)]),
@@ -442,12 +444,17 @@ def LBM_SetStreamingAccess(g: Grid, mode: int) -> None:
         raise ValueError(last_error())
 
 
+def LBM_SetBlockOrder(g: Grid, xcd_contiguous: bool) -> None:
+    if not load_library().LBM_SetBlockOrder(g.ptr, int(xcd_contiguous)):
+        raise ValueError(last_error())
+
+
 def LBM_GetKernelConfig(g: Grid):
-    """(workgroup size, streaming mode) of the step kernel."""
-    bs, nt = c_int(), c_int()
-    if not load_library().LBM_GetKernelConfig(g.ptr, ctypes.byref(bs), ctypes.byref(nt)):
+    """(workgroup size, streaming mode, XCD-contiguous order) of the step kernel."""
+    bs, nt, xo = c_int(), c_int(), c_int()
+    if not load_library().LBM_GetKernelConfig(g.ptr, ctypes.byref(bs), ctypes.byref(nt), ctypes.byref(xo)):
         raise RuntimeError(last_error())
-    return bs.value, nt.value
+    return bs.value, nt.value, bool(xo.value)
 
 
 def LBM_Autotune(g: Grid, steps: int, ux: float, uy: float = 0.0, uz: float = 0.0):
@@ -512,12 +519,17 @@ class Batch:
         if not load_library().LBM_BatchSetStreamingAccess(self.ptr, int(mode)):
             raise ValueError("invalid streaming mode (0-3) or batch not initialised")
 
+    def set_block_order(self, xcd_contiguous: bool):
+        if not load_library().LBM_BatchSetBlockOrder(self.ptr, int(xcd_contiguous)):
+            raise ValueError("batch not initialised")
+
     def kernel_config(self):
-        """(workgroup size, streaming mode) of the batch step kernel."""
-        bs, nt = c_int(), c_int()
-        if not load_library().LBM_BatchGetKernelConfig(self.ptr, ctypes.byref(bs), ctypes.byref(nt)):
+        """(workgroup size, streaming mode, XCD-contiguous order) of the batch step kernel."""
+        bs, nt, xo = c_int(), c_int(), c_int()
+        if not load_library().LBM_BatchGetKernelConfig(self.ptr, ctypes.byref(bs), ctypes.byref(nt),
+                                                       ctypes.byref(xo)):
             raise RuntimeError("batch not initialised")
-        return bs.value, nt.value
+        return bs.value, nt.value, bool(xo.value)
 
     @property
     def ntunnels(self) -> int:

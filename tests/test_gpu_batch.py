@@ -68,11 +68,12 @@ def test_graph_replay_autotune_and_block_sizes(gpu):
     for _ in range(24):  # eager single steps, no graph
         gpu.load_library().LBM_BatchStep(ref.ptr, 1)
     results = []
-    for bs, mode in ((128, 0), (512, 0), (256, 1), (512, 3), (128, 2)):
+    for bs, mode, xo in ((128, 0, True), (512, 0, False), (256, 1, True), (512, 3, False), (128, 2, False)):
         b = build()
         b.set_block_size(bs)
-        b.set_streaming(mode)  # cache policy only: the fields stay bit-identical
-        assert b.kernel_config() == (bs, mode)
+        b.set_streaming(mode)  # cache policy and workgroup order only:
+        b.set_block_order(xo)  # the fields stay bit-identical
+        assert b.kernel_config() == (bs, mode, xo)
         b.step(24)
         results.append(b)
     with pytest.raises(ValueError):

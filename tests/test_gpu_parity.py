@@ -444,9 +444,10 @@ def test_row_aligned_phase_b_is_bitwise_identical(gpu, op):
 
 @pytest.mark.parametrize("storage", [0, 1, 3])
 def test_streaming_access_and_autotune_leave_the_results_unchanged(gpu, storage):
-    """Non-temporal population loads/stores change only the cache policy: every
-    streaming mode at every workgroup size (row-aligned and per-cell phase-B
-    stores) gives bit-identical fields.  The autotuner, which times them,
+    """Non-temporal population loads/stores change only the cache policy, and
+    the workgroup order only which XCD runs which cells: every streaming mode
+    at every workgroup size (row-aligned and per-cell phase-B stores), in
+    both orders, gives bit-identical fields.  The autotuner, which times them,
     restores the state it started from, so a tuned run also matches bit for
     bit.  Modes outside 0-3 are rejected."""
     def run(bs, nt, tune=False):
@@ -467,7 +468,8 @@ def test_streaming_access_and_autotune_leave_the_results_unchanged(gpu, storage)
         else:
             gpu.LBM_SetBlockSize(g, bs)
             gpu.LBM_SetStreamingAccess(g, nt)
-            assert gpu.LBM_GetKernelConfig(g) == (bs, nt if storage != 3 else 0)
+            gpu.LBM_SetBlockOrder(g, bs != 256)
+            assert gpu.LBM_GetKernelConfig(g) == (bs, nt if storage != 3 else 0, bs != 256)
             with pytest.raises(ValueError):
                 gpu.LBM_SetStreamingAccess(g, 4)
         gpu.LBM_StepN(g, 6, 0.05, 0, 0)
