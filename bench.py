@@ -82,9 +82,12 @@ def setup_grid(lbm, geometry, w):
 
 def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
     """nsteps LBM steps in frames of 10 (particles advected after each frame).
-    With event_every = k > 0, the launches of every k-th frame are bracketed
-    by HIP events on the grid's stream (a recorded event pair idles the GPU
-    for several microseconds, so only a sample of the frames carries them).
+    Frames are queued without host synchronisation (LBM_StepN), as the
+    reference's main loop steps between samples.  With event_every = k > 0,
+    every k-th frame runs through LBM_ProfileSteps instead, whose launches
+    are bracketed by HIP events on the grid's stream and which synchronises
+    before and after (a recorded event pair idles the GPU for several
+    microseconds, so only a sample of the frames carries them).
     Returns (wall_s, main_ms_sum, boundary_ms_sum, timed_steps)."""
     nx = w["dims"][0]
     vel_scale = 8.0 / nx
@@ -97,11 +100,13 @@ def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
     while done < nsteps:
         n = min(10, nsteps - done)
         sample = event_every > 0 and frame % event_every == 0
-        _, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=sample)
         if sample:
+            _, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=True)
             main += mn
             bnd += bd
             timed += n
+        else:
+            lbm.LBM_StepN(g, n, w["u"], 0.0, 0.0)
         done += n
         frame += 1
         if parts is not None:
