@@ -192,7 +192,10 @@ def run(args):
         parts = lbm.DeviceParticles(w["particles"])
         lbm.LBM_SeedParticles(g, parts, 0)
 
-    tuned_block, tune_ms = lbm.LBM_Autotune(g, 20, w["u"], 0.0, 0.0)
+    # each of the 18 candidates is timed over about 2^31 cell updates (at least
+    # 20 steps), so a small lattice's choice does not rest on a millisecond
+    tune_steps = max(20, min(1000, (1 << 31) // cells))
+    tuned_block, tune_ms = lbm.LBM_Autotune(g, tune_steps, w["u"], 0.0, 0.0)
     _, streaming, xcd_order = lbm.LBM_GetKernelConfig(g)
     log(f"autotune: workgroup {tuned_block}, streaming mode {streaming}, XCD-contiguous order {xcd_order} "
         f"(best ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
