@@ -1,12 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the D3Q19 step on one MI355X.
 
-Workload (default): the 512x256x256 roofline configuration -- regularized
-collision + Smagorinsky (Cs 0.1), FP32 populations, tau 0.505, inlet u 0.05,
-periodic y/z, a seeded (seed 0) random ellipsoid-plus-box car proxy meshed
-with ~20k triangles and Bouzidi bounce-back, 50,000 particles seeded on the
-inlet plane from seed 0 and advected with RK2 trilinear sampling every 10
-LBM steps (the substep count of the reference's main loop).
+Workload (default): 512x256x256, MRT collision + Smagorinsky (Cs 0.1) -- the
+operator the reference's application runs (simulation/src/main.c:503-515) --
+FP32 populations, tau 0.505, inlet u 0.05, periodic y/z, a seeded (seed 0)
+random ellipsoid-plus-box car proxy meshed with ~20k triangles and Bouzidi
+bounce-back, 50,000 particles seeded on the inlet plane from seed 0 and
+advected with RK2 trilinear sampling every 10 LBM steps (the substep count of
+the reference's main loop).  The same lattice with regularized collision +
+Smagorinsky (the 512x256x256 roofline configuration of the workload list) is
+measured after it and reported under "secondary".
 
 A "step" is one LBM update of the whole lattice.  `value` is lattice updates
 per second of the whole timed region (inputs resident in HBM; includes the
@@ -47,7 +50,8 @@ WORKLOADS = {
     "512x256x256-regularized-smagorinsky-fp16": dict(
         dims=(512, 256, 256), op="reg", smag=1, tau=0.505, u=0.05, particles=50000, storage=1),
 }
-DEFAULT = "roofline-512x256x256-regularized-smagorinsky"
+DEFAULT = "512x256x256-mrt-smagorinsky"
+SECONDARY = "roofline-512x256x256-regularized-smagorinsky"
 
 
 def bytes_per_cell_storage(w):
@@ -138,20 +142,35 @@ def cpu_baseline(lbm, g, w, steps):
     cells = nx * ny * nz
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "logical_cpus": os.cpu_count(),
             "sample": f"{steps} steps of the {nx}x{ny}x{nz} {w['op']}+Smagorinsky workload "
                       f"({dt:.1f} s), oracle C/OpenMP restatement of the reference shaders, FP32, AoS two-buffer"}
 
 
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (None if unreadable)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def load_traffic(workload):
     """HBM bytes per launch of the step kernel from the committed rocprofv3 PMC
-    summary for this workload, if present."""
-    p = os.path.join(ROOT, "profiles", f"pmc_step_{workload}.json")
+    summary for this workload, if present: (bytes, source file) or (None, None).
+    The counters are not collected by this run (a --pmc pass is a separate
+    profiler run); the file holds the command that produced them."""
+    rel = os.path.join("profiles", f"pmc_step_{workload}.json")
+    p = os.path.join(ROOT, rel)
     if os.path.exists(p):
         try:
-            return json.load(open(p)).get("hbm_bytes_per_launch")
+            return json.load(open(p)).get("hbm_bytes_per_launch"), rel
         except Exception:
-            return None
-    return None
+            return None, None
+    return None, None
 
 
 def main():
@@ -161,6 +180,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--secondary", default=SECONDARY, choices=sorted(WORKLOADS) + ["none"],
+                    help="a second workload measured after the first and reported under 'secondary'")
     ap.add_argument("--event-every", type=int, default=10,
                     help="time the kernel launches of every k-th 10-step frame with HIP events (0 = none)")
     args = ap.parse_args()
@@ -180,10 +201,10 @@ def main():
     json_out.flush()
 
 
-def run(args):
-    from lattice_fluid_dynamics_amd import geometry, lbm
-
-    w = WORKLOADS[args.workload]
+def measure(lbm, geometry, name, args, cpu_steps):
+    """Set-up, autotuning, warm-up and the timed region of one workload; the
+    grid is freed on return.  Returns the fields of the JSON line."""
+    w = WORKLOADS[name]
     g, tris, bbox, area = setup_grid(lbm, geometry, w)
     nx, ny, nz = w["dims"]
     cells = nx * ny * nz
@@ -197,8 +218,8 @@ def run(args):
     tune_steps = max(20, min(1000, (1 << 31) // cells))
     tuned_block, tune_ms = lbm.LBM_Autotune(g, tune_steps, w["u"], 0.0, 0.0)
     _, streaming, xcd_order = lbm.LBM_GetKernelConfig(g)
-    log(f"autotune: workgroup {tuned_block}, streaming mode {streaming}, XCD-contiguous order {xcd_order} "
-        f"(best ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
+    log(f"{name}: autotune: workgroup {tuned_block}, streaming mode {streaming}, XCD-contiguous order "
+        f"{xcd_order} (best ms/step 128/256/512: {[round(float(v), 4) for v in tune_ms]})")
     if args.warmup:
         run_frames(lbm, g, parts, w, args.warmup, 0)
     wall, main_ms, bnd_ms, timed_steps = run_frames(lbm, g, parts, w, args.steps, 10 ** 6,
@@ -216,25 +237,29 @@ def run(args):
     if main_ms > 0:
         avg_s = main_ms / timed_steps / 1e3
         achieved = bpc * cells / avg_s / 1e9
+        traffic, traffic_src = load_traffic(name)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": (f"{traffic_src}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of a separate "
+                                   "profiler run, scaled by a copy kernel; not measured in this run")
+                if traffic_src else None,
                 "kernel": "k_step (fused stream+collide)", "bytes_per_cell": bpc,
                 "avg_launch_ms": round(main_ms / timed_steps, 5), "timed_launches": timed_steps,
                 "copy_kernel_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
                 "workgroup": tuned_block, "streaming_mode": streaming, "xcd_contiguous": xcd_order,
                 "boundary_prepass_avg_ms": round(bnd_ms / max(timed_steps, 1), 5)}
     cpu = None
-    if args.cpu_steps > 0:
+    if cpu_steps > 0:
         log("timing the CPU oracle ...")
-        cpu = cpu_baseline(lbm, g, w, args.cpu_steps)
-
+        cpu = cpu_baseline(lbm, g, w, cpu_steps)
+    if parts is not None:
+        parts.free()
+    lbm.LBM_Free(g)
     storage = {0: "f32", 1: "f16 (shifted storage, f32 math)", 2: "bf16 (shifted storage, f32 math)", 3: "f64"}
-    out = {
-        "metric": "MLUPS (D3Q19 lattice updates per second; % of HBM roofline at 152 B/cell/step FP32)",
-        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": storage[w["storage"]], "data": "synthetic",
-        "config": {"workload": args.workload, "lattice": f"{nx}x{ny}x{nz}",
+    return {
+        "value": round(mlups, 1), "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "dtype": storage[w["storage"]],
+        "config": {"workload": name, "lattice": f"{nx}x{ny}x{nz}",
                    "collision": {"reg": "regularized", "mrt": "MRT"}[w["op"]] + "+Smagorinsky(Cs=0.1)",
                    "tau": w["tau"], "inlet_u": w["u"], "periodicYZ": 1,
                    "body": f"car proxy seed 0, {len(tris)} triangles, Bouzidi",
@@ -244,6 +269,29 @@ def run(args):
         "cd": round(abs(fx) / dyn, 4) if dyn > 0 else None,
         "hbm_roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpc / 1e6, 1),
     }
+
+
+def run(args):
+    from lattice_fluid_dynamics_amd import geometry, lbm
+
+    m = measure(lbm, geometry, args.workload, args, args.cpu_steps)
+    out = {
+        "metric": "MLUPS (D3Q19 lattice updates per second; % of HBM roofline at 152 B/cell/step FP32)",
+        "value": m["value"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": m["dtype"], "data": "synthetic",
+        "config": m["config"], "roofline": m["roofline"], "cpu_baseline": m["cpu_baseline"],
+        "cd": m["cd"], "hbm_roofline_mlups": m["hbm_roofline_mlups"],
+    }
+    if args.secondary != "none" and args.secondary != args.workload:
+        s = measure(lbm, geometry, args.secondary, args, 0)
+        r = s["roofline"] or {}
+        out["secondary"] = {
+            "workload": args.secondary, "value": s["value"], "unit": "MLUPS", "ms_per_step": s["ms_per_step"],
+            "collision": s["config"]["collision"], "roofline_frac": r.get("frac"),
+            "avg_launch_ms": r.get("avg_launch_ms"), "achieved_gbs": r.get("achieved"),
+            "traffic": r.get("traffic"), "traffic_source": r.get("traffic_source"),
+            "workgroup": r.get("workgroup"), "streaming_mode": r.get("streaming_mode"), "cd": s["cd"]}
     return out
 
 

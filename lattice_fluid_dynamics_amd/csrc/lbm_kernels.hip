@@ -547,6 +547,28 @@ __device__ __forceinline__ double wave_shift1(double v) {
     const int shi = __builtin_amdgcn_update_dpp(hi, hi, UP ? 0x138 : 0x130, 0xF, 0xF, false);
     return __longlong_as_double(((long long)shi << 32) | (unsigned)slo);
 }
+template <bool UP>
+__device__ __forceinline__ uint16_t wave_shift1(uint16_t v) {
+    const int i = (int)v;
+    return (uint16_t)__builtin_amdgcn_update_dpp(i, i, UP ? 0x138 : 0x130, 0xF, 0xF, false);
+}
+
+// The same uniform pointer, opaque to the compiler (an empty asm statement
+// that claims to modify it).  Plane resources derived from it are then
+// rebuilt where they are used instead of being kept live in scalar registers
+// across the collision, which would cost a wave per SIMD of occupancy.
+template <typename S>
+__device__ __forceinline__ S *opaque_ptr(S *p) {
+    const uint64_t v = (uint64_t)p;
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    asm volatile("" : "+s"(lo), "+s"(hi));
+    return (S *)(((uint64_t)hi << 32) | lo);
+}
+
+// Position of direction i in XPLUS (ex = +1) or XMINUS (ex = -1).
+__host__ __device__ constexpr int xdir_slot(int i) {
+    return (i == 1 || i == 2) ? 0 : (i == 7 || i == 8) ? 1 : (i == 9 || i == 10) ? 2 : (i == 11 || i == 12) ? 3 : 4;
+}
 
 template <int FMT, int OP, int PHASE, int ROW = 0, int NT = 0>
 __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell) {
@@ -564,8 +586,55 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
     const uint8_t m = a.mask[cell];
     const bool ovr = a.use_ovr && (m & LBMDEV_MASK_OVR);
     const SlotIO<typename P::S, NT ? CP_NT : 0> io(a.pop, g);
+    // Row geometry for ROW = 1: wave w of the workgroup covers row positions
+    // [64 rw, 64 rw + 64) of its row; wprev / wnext are the row's neighbouring
+    // waves, wrapping at the row ends (x is periodic on the torus).
+    // The wave index is uniform: readfirstlane keeps it and its neighbours
+    // in scalar registers.
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpr = ROW ? g.nx >> 6 : 1, rw = w % wpr, w0 = w - rw;
+    const int wprev = rw == 0 ? w0 + wpr - 1 : w - 1, wnext = rw == wpr - 1 ? w0 : w + 1;
     T f[19];
-    if (ovr) {
+    if (PHASE == 1 && ROW) {
+        // Row-aligned phase-B loads, the mirror of the row-aligned stores
+        // below.  Cell x needs slot (opp(i), x - e_i).  Lane l of a wave
+        // starting at x0 loads slot (opp(i), x0 + l) at the y/z offset of
+        // direction i, so every load covers whole cache lines, and the value
+        // moves to the lane that needs it: for ex_i = +1 from lane l - 1, for
+        // ex_i = -1 from lane l + 1 (DPP), the wave-edge values through LDS
+        // from the neighbouring wave of the row.  Every lane loads, listed
+        // cells included, because its neighbours need the values; listed
+        // cells then take their incoming values from the boundary list.
+        typedef typename P::S S;
+        __shared__ S lp[8][5], lm[8][5];
+        S raw[19];
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+            raw[i] = io.ld(OPP[i], (uint32_t)(X[1] + Y[1 - EY[i]] + Z[1 - EZ[i]]));
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            if (lane == 63) lp[w][k] = raw[XPLUS[k]];
+            if (lane == 0) lm[w][k] = raw[XMINUS[k]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            S v = raw[i];
+            if (EX[i] == 1) {
+                v = wave_shift1<true>(v);
+                if (lane == 0) v = lp[wprev][xdir_slot(i)];
+            } else if (EX[i] == -1) {
+                v = wave_shift1<false>(v);
+                if (lane == 63) v = lm[wnext][xdir_slot(i)];
+            }
+            f[i] = P::dec(v, OPP[i]);
+        }
+        if (ovr) {
+            const int k = a.ovr_index[cell];
+#pragma unroll
+            for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
+        }
+    } else if (ovr) {
         const int k = a.ovr_index[cell];
 #pragma unroll
         for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
@@ -600,10 +669,20 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
             io.st(j, (uint32_t)cell, P::enc(fo[i], j));
         }
     } else if (!ROW) {
+        // Offsets recomputed from an opaque copy of the cell index (see the
+        // row-aligned branch below).
+        int c2 = cell;
+        asm volatile("" : "+v"(c2));
+        int x2, y2, z2;
+        decompose(g, c2, x2, y2, z2);
+        const int Xs[3] = {x2 == 0 ? g.nx - 1 : x2 - 1, x2, x2 == g.nx - 1 ? 0 : x2 + 1};
+        const int Ys[3] = {(y2 == 0 ? g.ny - 1 : y2 - 1) * g.nx, y2 * g.nx, (y2 == g.ny - 1 ? 0 : y2 + 1) * g.nx};
+        const int Zs[3] = {(z2 == 0 ? g.nz - 1 : z2 - 1) * pl, z2 * pl, (z2 == g.nz - 1 ? 0 : z2 + 1) * pl};
+        const SlotIO<typename P::S, NT ? CP_NT : 0> ios(opaque_ptr(a.pop), g);
 #pragma unroll
         for (int i = 0; i < 19; i++) {
-            const uint32_t dst = (uint32_t)(X[1 + EX[i]] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
-            io.st(i, dst, P::enc(fo[i], i));
+            const uint32_t dst = (uint32_t)(Xs[1 + EX[i]] + Ys[1 + EY[i]] + Zs[1 + EZ[i]]);
+            ios.st(i, dst, P::enc(fo[i], i));
         }
     } else {
         // Row-aligned phase-B stores (the workgroup holds whole x-rows, nx a
@@ -616,9 +695,16 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         // whole cache lines.  Values move between lanes by whole-wave DPP
         // shifts; the wave-edge values come through LDS.
         __shared__ T xp[8][5], xm[8][5];
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        const int wpr = g.nx >> 6, rw = w % wpr, w0 = w - rw;
-        const int wprev = rw == 0 ? w0 + wpr - 1 : w - 1, wnext = rw == wpr - 1 ? w0 : w + 1;
+        // The store offsets are recomputed from an opaque copy of the cell
+        // index rather than kept live from the loads (whose y/z offsets are
+        // the same nine), which would cost a wave per SIMD of occupancy.
+        const SlotIO<typename P::S, NT ? CP_NT : 0> ios(opaque_ptr(a.pop), g);
+        int c2 = cell;
+        asm volatile("" : "+v"(c2));
+        int x2, y2, z2;
+        decompose(g, c2, x2, y2, z2);
+        const int Ys[3] = {(y2 == 0 ? g.ny - 1 : y2 - 1) * g.nx, y2 * g.nx, (y2 == g.ny - 1 ? 0 : y2 + 1) * g.nx};
+        const int Zs[3] = {(z2 == 0 ? g.nz - 1 : z2 - 1) * pl, z2 * pl, (z2 == g.nz - 1 ? 0 : z2 + 1) * pl};
 #pragma unroll
         for (int k = 0; k < 5; k++) {
             if (lane == 63) xp[w][k] = fo[XPLUS[k]];
@@ -627,29 +713,24 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 19; i++) {
-            const uint32_t dst = (uint32_t)(X[1] + Y[1 + EY[i]] + Z[1 + EZ[i]]);
+            const uint32_t dst = (uint32_t)(x2 + Ys[1 + EY[i]] + Zs[1 + EZ[i]]);
             T v = fo[i];
             if (EX[i] == 1) {
                 v = wave_shift1<true>(v);
-                int k = 0;
-#pragma unroll
-                for (int j = 0; j < 5; j++) k = XPLUS[j] == i ? j : k;
-                if (lane == 0) v = xp[wprev][k];
+                if (lane == 0) v = xp[wprev][xdir_slot(i)];
             } else if (EX[i] == -1) {
                 v = wave_shift1<false>(v);
-                int k = 0;
-#pragma unroll
-                for (int j = 0; j < 5; j++) k = XMINUS[j] == i ? j : k;
-                if (lane == 63) v = xm[wnext][k];
+                if (lane == 63) v = xm[wnext][xdir_slot(i)];
             }
-            io.st(i, dst, P::enc(v, i));
+            ios.st(i, dst, P::enc(v, i));
         }
     }
 }
 
 // The fused kernel over one lattice, BS lanes per workgroup.
-// ROW = 1 (phase B only): row-aligned stores; the launch guarantees that
-// every workgroup holds whole rows and no lane is past the lattice.
+// ROW = 1 (phase B only): row-aligned loads and stores; the launch
+// guarantees that every workgroup holds whole rows and no lane is past the
+// lattice.
 // NT = 1: non-temporal population loads and stores.
 template <int FMT, int OP, int PHASE, int BS, int ROW = 0, int NT = 0>
 __global__ __launch_bounds__(BS) void k_step(StepArgs<FMT> a) {

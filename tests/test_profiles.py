@@ -36,4 +36,6 @@ def test_pmc_summary_is_consistent(name):
     # HBM traffic can exceed the algorithmic bytes (mask, list, velocity
     # output, partial lines) but not fall below what the populations need
     assert 1.0 <= p["traffic_over_algorithmic"] < 1.15
-    assert bench.load_traffic(workload) == p["hbm_bytes_per_launch"]
+    traffic, source = bench.load_traffic(workload)
+    assert traffic == p["hbm_bytes_per_launch"]
+    assert source == os.path.join("profiles", name)
