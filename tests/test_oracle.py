@@ -244,6 +244,77 @@ def test_sphere_q_lies_on_the_surface():
     assert np.array_equal(g.q >= 0, links)
 
 
+# The ez = -1 directions of D3Q19: (0,0,-1), (+-1,0,-1), (0,+-1,-1).
+DOWN = [i for i in range(19) if E[i, 2] == -1]
+
+
+@pytest.mark.parametrize("dims,layer", [((16, 8, 8), 0), ((32, 16, 16), 3), ((20, 12, 24), 7), ((10, 6, 8), 6)])
+def test_ground_plane_closed_form(dims, layer):
+    """simulation/src/lbm.c:537-614 on an empty grid.  worldZ is chosen in the
+    middle of lattice layer `layer` (z = -2 + 4 (layer + 1/2) / nz), so the
+    ground holds exactly the (layer + 1) lowest layers: nx ny (layer + 1)
+    cells of 2.  The links are q = 1/2 on the five ez = -1 directions of
+    every cell of the next layer whose target lies inside the x/y extent:
+    5 nx ny minus one row per edge direction, 5 nx ny - 2 nx - 2 ny in all.
+    Every other q stays -1."""
+    nx, ny, nz = dims
+    assert sorted(DOWN) == [6, 13, 14, 17, 18]
+    g = O.OracleGrid(nx, ny, nz, 0.05)
+    g.add_ground_plane(-2.0 + 4.0 * (layer + 0.5) / nz)
+    s = g.solid.reshape(nz, ny, nx)
+    assert np.all(s[: layer + 1] == 2) and np.all(s[layer + 1:] == 0)
+    q = g.q.reshape(nz, ny, nx, 19)
+    linked = np.argwhere(q >= 0)
+    last = layer + 1 == nz  # the whole lattice is ground: no layer above
+    assert len(linked) == (0 if last else 5 * nx * ny - 2 * nx - 2 * ny)
+    assert np.all(q[q >= 0] == 0.5)
+    for z, y, x, i in linked:
+        assert z == layer + 1 and i in DOWN
+        assert 0 <= x + E[i, 0] < nx and 0 <= y + E[i, 1] < ny
+
+
+def test_ground_plane_clamps_and_keeps_bodies():
+    """The ground layer index is clamped to [0, nz - 1]; body cells (1) keep
+    their value; a link q already set by the body (the analytic sphere q,
+    not 1/2) is kept; the plane adds links only in the layer above the
+    ground, only on ez = -1 directions and only towards non-fluid cells."""
+    nx, ny, nz = 32, 16, 16
+    g = O.OracleGrid(nx, ny, nz, 0.05)
+    g.add_ground_plane(-9.0)  # below the box: layer 0
+    assert np.all(g.solid.reshape(nz, ny, nx)[0] == 2) and np.count_nonzero(g.solid) == nx * ny
+    g = O.OracleGrid(nx, ny, nz, 0.05)
+    g.add_ground_plane(9.0)  # above the box: every cell is ground, no links
+    assert np.all(g.solid == 2) and np.all(g.q < 0)
+
+    g = O.OracleGrid(nx, ny, nz, 0.05)
+    g.set_solid_sphere(0.0, 0.0, -1.3, 0.6)  # reaches below z = -1.5
+    solid0, q0 = g.solid.copy(), g.q.copy()
+    g.add_ground_plane(-1.5)
+    layer = int((-1.5 + 2.0) * nz / 4.0)
+    s0, s = solid0.reshape(nz, ny, nx), g.solid.reshape(nz, ny, nx)
+    assert np.all(s[s0 == 1] == 1)
+    assert np.array_equal(s[: layer + 1] != 0, np.ones_like(s[: layer + 1], bool))
+    assert np.all(s[: layer + 1][s0[: layer + 1] == 0] == 2)
+    assert np.array_equal(s[layer + 1:], s0[layer + 1:])
+    kept = q0 >= 0
+    assert np.array_equal(g.q[kept], q0[kept])
+    new = np.argwhere((g.q >= 0) & ~kept)
+    assert len(new) > 0
+    q = g.q.reshape(nz, ny, nx, 19)
+    for c, i in new:
+        x, y, z = c % nx, (c // nx) % ny, c // (nx * ny)
+        assert z == layer + 1 and i in DOWN and q[z, y, x, i] == 0.5 and s[z, y, x] == 0
+        assert s[z - 1, y + E[i, 1], x + E[i, 0]] != 0
+    # fluid cells of that layer whose ez = -1 target is ground got a link
+    for y in range(ny):
+        for x in range(nx):
+            if s[layer + 1, y, x] == 0:
+                for i in DOWN:
+                    X, Y = x + E[i, 0], y + E[i, 1]
+                    if 0 <= X < nx and 0 <= Y < ny:
+                        assert q[layer + 1, y, x, i] >= 0
+
+
 def box_mesh(lo, hi):
     """12 triangles (12 floats each: v0 xyz pad, v1 xyz pad, v2 xyz pad) of an axis box."""
     x0, y0, z0 = lo
