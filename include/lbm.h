@@ -186,6 +186,13 @@ int LBM_ProfileSteps(LBMGrid *grid, int n, float inletVelX, float inletVelY, flo
 /* Number of cells in the compacted boundary list used by the prepass. */
 int LBM_GetBoundaryCellCount(LBMGrid *grid);
 
+/* No reference counterpart (diagnostics for long runs): the number of cell
+ * updates the stability guard of lbm_collide.comp:429-440 reset to rest
+ * (rho < 0.3 or > 2, |u| > 0.4, or a non-finite rho) since the last
+ * LBM_InitializeFlow*, counted on the device; LBM_Autotune's trials do not
+ * count.  Synchronises the grid's stream; -1 on error. */
+long long LBM_GetGuardResetCount(LBMGrid *grid);
+
 /* Text of the last device or argument error ("" if none). */
 const char *LBM_GetLastError(void);
 
@@ -299,6 +306,8 @@ int LBM_BatchSetStreamingAccess(LBMBatch *batch, int mode);
 int LBM_BatchSetBlockOrder(LBMBatch *batch, int xcdContiguous);
 int LBM_BatchGetKernelConfig(const LBMBatch *batch, int *blockSize, int *streamingMode, int *xcdContiguous);
 int LBM_BatchTunnelCount(const LBMBatch *batch);
+/* As LBM_GetGuardResetCount, summed over all tunnels of the batch. */
+long long LBM_BatchGuardResetCount(LBMBatch *batch);
 long long LBM_BatchTotalCells(const LBMBatch *batch);
 /* out: 6 doubles per tunnel (F xyz, pressure part xyz, FP64 sums); fixedOut
  * (may be NULL): the 7 reference fixed-point accumulators per tunnel. */

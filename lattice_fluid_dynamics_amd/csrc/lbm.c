@@ -511,8 +511,19 @@ const int32_t *lbm_host_solid(const LBMGrid *g) { return ((const Impl *)g->impl)
 LbmDev *lbm_device_of(LBMGrid *g) { return impl_of(g)->dev; }
 
 /* Builds the boundary list for the current periodicYZ and uploads geometry. */
+/* The grid's device state, or NULL (with an error noted) when an
+ * LBM_InitializeFlow that switched the storage format could allocate neither
+ * the new nor the previous format.  Every call that reaches the device goes
+ * through this check. */
+static LbmDev *live_dev(LBMGrid *g) {
+    LbmDev *d = impl_of(g)->dev;
+    if (!d) note_error("the grid has no device state: an earlier LBM_InitializeFlow failed to allocate it");
+    return d;
+}
+
 static int ensure_geometry(LBMGrid *g) {
     Impl *im = impl_of(g);
+    if (!live_dev(g)) return 0;
     const int periodic = g->periodicYZ == 1;
     if (!im->geomDirty && im->builtPeriodic == periodic) return 1;
     const int n = lbm_boundary_count(g, periodic);
@@ -543,11 +554,22 @@ int LBM_GetBoundaryCellCount(LBMGrid *g) {
     return impl_of(g)->nBoundary;
 }
 
+long long LBM_GetGuardResetCount(LBMGrid *g) {
+    long long n = -1;
+    if (!g || !live_dev(g)) return -1;
+    if (!lbmdev_guard_resets(impl_of(g)->dev, &n)) {
+        note_error(lbmdev_last_error());
+        return -1;
+    }
+    return n;
+}
+
 /* ------------------------------------------------------------ flow */
 
 /* Device buffers in the requested storage format and the geometry uploaded. */
 static int prepare_flow(LBMGrid *g) {
     Impl *im = impl_of(g);
+    if (!live_dev(g)) return 0;
     if (g->populationStorage != lbmdev_storage(im->dev)) {
         char err[256] = {0};
         const int previous = lbmdev_storage(im->dev);
@@ -561,10 +583,9 @@ static int prepare_flow(LBMGrid *g) {
             char err2[256] = {0};
             g->populationStorage = previous;
             im->dev = lbmdev_create(g->sizeX, g->sizeY, g->sizeZ, previous, err2, sizeof(err2));
-            if (im->dev) {
-                g->velocityBuffer = lbmdev_velocity_ptr(im->dev);
-                g->solidBuffer = lbmdev_solid_ptr(im->dev);
-            }
+            g->velocityBuffer = im->dev ? lbmdev_velocity_ptr(im->dev) : NULL;
+            g->solidBuffer = im->dev ? lbmdev_solid_ptr(im->dev) : NULL;
+            im->flowReady = 0; /* the populations are gone either way */
             im->geomDirty = 1;
             return 0;
         }
@@ -654,7 +675,7 @@ int LBM_ProfileSteps(LBMGrid *g, int n, float ux, float uy, float uz, float *tot
 float *LBM_GetVelocityBuffer(LBMGrid *g) { return g ? g->velocityBuffer : NULL; }
 
 void LBM_Synchronize(LBMGrid *g) {
-    if (g) lbmdev_synchronize(impl_of(g)->dev);
+    if (g && live_dev(g)) lbmdev_synchronize(impl_of(g)->dev);
 }
 
 /* ------------------------------------------------------------ force */
@@ -731,7 +752,7 @@ float LBM_ComputeLiftCoefficient(LBMGrid *g, float U, float A) {
 /* ------------------------------------------------------------ readback */
 
 int LBM_CopyVelocityToHost(LBMGrid *g, float *out) {
-    if (!g || !out) return 0;
+    if (!g || !out || !live_dev(g)) return 0;
     return lbmdev_copy_velocity(impl_of(g)->dev, out);
 }
 
@@ -740,7 +761,7 @@ int LBM_CopyCoarseVelocityToHost(LBMGrid *g, int factor, float *out) {
         note_error("LBM_CopyCoarseVelocityToHost: the grid must be a multiple of the factor on every axis");
         return 0;
     }
-    if (!lbmdev_copy_velocity_coarse(impl_of(g)->dev, factor, out)) {
+    if (!live_dev(g) || !lbmdev_copy_velocity_coarse(impl_of(g)->dev, factor, out)) {
         note_error(lbmdev_last_error());
         return 0;
     }
@@ -782,13 +803,13 @@ double LBM_ComputeTotalMass(LBMGrid *g) {
 /* ------------------------------------------------------------ particles */
 
 void LBM_SeedParticles(LBMGrid *g, Particle *p, int n, unsigned int seed) {
-    if (!g || !p) return;
+    if (!g || !p || !live_dev(g)) return;
     if (!lbmdev_particles_seed(impl_of(g)->dev, p, n, seed)) note_error(lbmdev_last_error());
 }
 
 void LBM_AdvectParticles(LBMGrid *g, Particle *p, int n, float dt, float velScale, unsigned int seed,
                          unsigned int step) {
-    if (!g || !p) return;
+    if (!g || !p || !live_dev(g)) return;
     if (!lbmdev_particles_advect(impl_of(g)->dev, p, n, dt, velScale, seed, step))
         note_error(lbmdev_last_error());
 }

@@ -256,6 +256,12 @@ int LBM_BatchGetKernelConfig(const LBMBatch *b, int *blockSize, int *streamingMo
 
 int LBM_BatchTunnelCount(const LBMBatch *b) { return b ? b->nt : 0; }
 
+long long LBM_BatchGuardResetCount(LBMBatch *b) {
+    long long n = -1;
+    if (!b || !b->dev || !lbmbatch_guard_resets(b->dev, &n)) return -1;
+    return n;
+}
+
 long long LBM_BatchTotalCells(const LBMBatch *b) {
     long long c = 0;
     for (int k = 0; b && k < b->nt; k++) c += (long long)b->t[k].nx * b->t[k].ny * b->t[k].nz;

@@ -124,6 +124,10 @@ int lbmbatch_export(LbmBatchDev *b, int t, int periodic, double *f_post_stream, 
 int lbmbatch_copy_velocity(LbmBatchDev *b, int t, float *host);
 int lbmbatch_copy_velocity_coarse(LbmBatchDev *b, int t, int f, float *host);
 int lbmbatch_synchronize(LbmBatchDev *b);
+/* Cells reset by the stability guard since the flow was initialised (all
+ * tunnels of a batch together); 1 on success. */
+int lbmdev_guard_resets(LbmDev *d, long long *count);
+int lbmbatch_guard_resets(LbmBatchDev *b, long long *count);
 
 /* Read + write bandwidth of a plain copy of `bytes` bytes (mean of iters). */
 int lbmdev_copy_bandwidth(size_t bytes, int iters, float *gbps, float *ms_per_copy);

@@ -321,12 +321,14 @@ __device__ __forceinline__ bool speed_exceeds<double>(double u2) { return sqrt(u
 struct CollideParams {
     float tau, cs2x18, inx, iny, inz;
     int smag;
+    unsigned long long *guard;  // running count of stability-guard resets
 };
 
 // lbm_collide.comp:313-626 for one non-body cell: Zou-He, moments, guard,
 // Smagorinsky, then regularized / MRT / BGK.  f is updated by the Zou-He
-// rewrite; fo receives the post-collision populations; returns the vec4
-// written to the velocity buffer.
+// rewrite; fo receives the post-collision populations and rho, u the values
+// written to the velocity buffer.  A reset by the stability guard is counted
+// in p.guard.
 template <int OP, typename T>
 __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollideParams &p, int x, int nx,
                                               T &rho, T &ux, T &uy, T &uz) {
@@ -375,6 +377,7 @@ __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollidePa
         for (int i = 0; i < 19; i++) fo[i] = wgt<T>(i);
         rho = T(1);
         ux = uy = uz = T(0);
+        atomicAdd(p.guard, 1ull);  // one atomic per reset cell; the guard normally never fires
         return;
     }
     const T u2 = ux * ux + uy * uy + uz * uz;
@@ -660,6 +663,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
     } else {
         collide_fluid<OP, T>(f, fo, a.cp, x, g.nx, rho, ux, uy, uz);
     }
+
     if (a.write_vel) a.vel[cell] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
 
     if (PHASE == 0) {
@@ -765,6 +769,7 @@ struct BatchArgs {
     float cs2x18;
     int smag;
     int remap;  // 1: XCD-contiguous workgroup order (xcd_remap)
+    unsigned long long *guard;  // running count of stability-guard resets
     const TunnelDesc *desc;
     const int32_t *block_tunnel;
 };
@@ -793,6 +798,7 @@ __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     a.cp.iny = d.iny;
     a.cp.inz = d.inz;
     a.cp.smag = b.smag;
+    a.cp.guard = b.guard;
     step_cell<FMT, OP, PHASE, ROW, NT>(a, cell);
 }
 
@@ -1267,6 +1273,7 @@ struct LbmDev {
     long long *pi;
     double *resd;
     long long *resi;
+    unsigned long long *guard;  // stability-guard resets since the flow was initialised
     float *feq_d;
     int nparts;  // partial slots allocated
     int bs;      // workgroup size of the step kernel
@@ -1334,6 +1341,7 @@ extern "C" LbmDev *lbmdev_create(int nx, int ny, int nz, int storage, char *err,
     ALLOC(d->resd, 16 * 8, "reduction result");
     ALLOC(d->resi, 16 * 8, "reduction result");
     ALLOC(d->feq_d, 19 * 4, "feq constants");
+    ALLOC(d->guard, 8, "guard counter");
 #undef ALLOC
     if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) {
         snprintf(err, errlen, "stream creation failed: %s", hipGetErrorString(e));
@@ -1368,6 +1376,7 @@ extern "C" void lbmdev_destroy(LbmDev *d) {
     hipFree(d->resd);
     hipFree(d->resi);
     hipFree(d->feq_d);
+    hipFree(d->guard);
     if (d->stream) hipStreamDestroy(d->stream);
     free(d);
 }
@@ -1438,6 +1447,7 @@ static int init_t(LbmDev *d, const float4 &v4) {
 
 extern "C" int lbmdev_init_flow(LbmDev *d, const float feq[19], const float vel4[4]) {
     HIPCHECK(hipMemcpyAsync(d->feq_d, feq, 19 * 4, hipMemcpyHostToDevice, d->stream), "feq upload");
+    HIPCHECK(hipMemsetAsync(d->guard, 0, 8, d->stream), "guard counter");
     float4 v4 = make_float4(vel4[0], vel4[1], vel4[2], vel4[3]);
     switch (d->fmt) {
     case LBMDEV_FP32: init_t<LBMDEV_FP32>(d, v4); break;
@@ -1476,6 +1486,7 @@ extern "C" int lbmdev_init_field(LbmDev *d, const float *coarse_host, int cnx, i
     float *c = NULL;
     HIPCHECK(hipMalloc((void **)&c, bytes), "coarse field");
     hipError_t e = hipMemcpy(c, coarse_host, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(d->guard, 0, 8, d->stream);
     if (e == hipSuccess) {
         init_field_dispatch(d->fmt, d->pop, make_geo(d->nx, d->ny, d->nz, d->stride), d->vel, c, cnx, cny, cnz,
                             d->stream);
@@ -1555,6 +1566,7 @@ static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ov
     a.cp.inz = p->inletZ;
     a.cp.smag = p->useSmagorinsky == 1;
     a.remap = d->remap;
+    a.cp.guard = d->guard;
     // FP64 storage uses global addressing and has no streaming variant
     if (FMT != LBMDEV_FP64 && (d->nt & (phase == 0 ? LBMDEV_NT_A : LBMDEV_NT_B)))
         launch_step_bs<FMT, OP, FMT != LBMDEV_FP64>(d->bs, phase, d->ncell, d->stream, a);
@@ -1933,6 +1945,7 @@ struct LbmBatchDev {
     long long *pi;
     double *resd;
     long long *resi;
+    unsigned long long *guard;  // stability-guard resets over all tunnels
     int nparts;
     hipGraphExec_t graph[2];
     int graph_ok;
@@ -2039,6 +2052,7 @@ extern "C" LbmBatchDev *lbmbatch_create(int storage, int nt, const int *dims, co
     BALLOC(b->pi, (size_t)b->nparts * 8 * 8, "reduction partials");
     BALLOC(b->resd, (size_t)nt * 8 * 8, "reduction results");
     BALLOC(b->resi, (size_t)nt * 8 * 8, "reduction results");
+    BALLOC(b->guard, 8, "guard counter");
 #undef BALLOC
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
         snprintf(err, errlen, "stream creation failed: %s", hipGetErrorString(e));
@@ -2071,6 +2085,7 @@ extern "C" void lbmbatch_destroy(LbmBatchDev *b) {
     hipFree(b->pi);
     hipFree(b->resd);
     hipFree(b->resi);
+    hipFree(b->guard);
     if (b->stream) hipStreamDestroy(b->stream);
     free(b->desc_h);
     free(b->dims);
@@ -2155,6 +2170,7 @@ extern "C" int lbmbatch_init_flow(LbmBatchDev *b, const float *feq, const float 
     float *feq_d = NULL;
     HIPCHECK(hipMalloc((void **)&feq_d, sizeof(float) * 19 * (size_t)b->nt), "feq table");
     hipError_t e = hipMemcpy(feq_d, feq, sizeof(float) * 19 * (size_t)b->nt, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(b->guard, 0, 8, b->stream);
     if (e == hipSuccess) {
         switch (b->fmt) {
         case LBMDEV_FP32: batch_init_t<LBMDEV_FP32>(b, feq_d, vel4); break;
@@ -2211,6 +2227,7 @@ static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase
     a.desc = b->desc_d;
     a.block_tunnel = b->block_tunnel;
     a.remap = b->remap;
+    a.guard = b->guard;
     // row-aligned phase B when every tunnel's rows fill whole workgroups
     bool row = FMT != LBMDEV_FP64;
     for (int t = 0; t < b->nt && row; t++) {
@@ -2558,8 +2575,36 @@ static int snapshot_put(const Snapshot &s, void *pop, void *vel, hipStream_t st)
     return 1;
 }
 
+// The guard counter around trials that do not advance the run.
+static int guard_read(const unsigned long long *g, unsigned long long *v, hipStream_t st) {
+    HIPCHECK(hipMemcpyAsync(v, g, 8, hipMemcpyDeviceToHost, st), "guard counter");
+    HIPCHECK(hipStreamSynchronize(st), "guard counter");
+    return 1;
+}
+static int guard_write(unsigned long long *g, unsigned long long v, hipStream_t st) {
+    HIPCHECK(hipMemcpyAsync(g, &v, 8, hipMemcpyHostToDevice, st), "guard counter");
+    HIPCHECK(hipStreamSynchronize(st), "guard counter");
+    return 1;
+}
+
+extern "C" int lbmdev_guard_resets(LbmDev *d, long long *count) {
+    unsigned long long v = 0;
+    if (!guard_read(d->guard, &v, d->stream)) return 0;
+    *count = (long long)v;
+    return 1;
+}
+
+extern "C" int lbmbatch_guard_resets(LbmBatchDev *b, long long *count) {
+    unsigned long long v = 0;
+    if (!guard_read(b->guard, &v, b->stream)) return 0;
+    *count = (long long)v;
+    return 1;
+}
+
 extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, float *ms_per_step) {
     if (d->state < 0 || steps <= 0) return 0;
+    unsigned long long guard0 = 0;
+    if (!guard_read(d->guard, &guard0, d->stream)) return 0;
     Snapshot s;
     const size_t pb = (size_t)19 * d->stride * d->elem, vb = (size_t)d->ncell * 16;
     if (!snapshot_take(s, d->pop, pb, d->vel, vb, d->stream)) return 0;
@@ -2593,11 +2638,14 @@ extern "C" int lbmdev_autotune(LbmDev *d, const LbmStepParams *p, int steps, flo
     d->bs = ok ? best : bs0;
     d->nt = ok ? bestNt : nt0;
     d->remap = ok ? bestRemap : remap0;
+    if (!guard_write(d->guard, guard0, d->stream)) return 0;
     return ok ? best : 0;
 }
 
 extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int steps, float *ms_per_step) {
     if (b->state < 0 || steps <= 0) return 0;
+    unsigned long long guard0 = 0;
+    if (!guard_read(b->guard, &guard0, b->stream)) return 0;
     Snapshot s;
     const size_t pb = (size_t)b->total_pop * b->elem, vb = (size_t)b->total_cells * 16;
     if (!snapshot_take(s, b->pop, pb, b->vel, vb, b->stream)) return 0;
@@ -2629,6 +2677,7 @@ extern "C" int lbmbatch_autotune(LbmBatchDev *b, const LbmBatchParams *p, int st
     hipFree(s.vel);
     b->smode = ok ? bestMode : smode0;
     b->remap = ok ? bestRemap : remap0;
+    if (!guard_write(b->guard, guard0, b->stream)) return 0;
     if (!lbmbatch_set_block_size(b, ok ? best : bs0)) return 0;
     return ok ? best : 0;
 }

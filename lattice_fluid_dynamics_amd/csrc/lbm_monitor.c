@@ -118,7 +118,9 @@ int LBM_CdMonitorSample(LBMCdMonitor *m, LBMGrid *grid, float out[6]) {
         print_breakdown(m);
         m->breakdownPrinted = 1;
     }
-    if (m->count == 0 && !m->converged && m->clCount == 0)
+    /* main.c:1309-1314: the EMA restarts from the sample itself while no Cd
+     * has entered the convergence history yet (only 0 < Cd < 1000 does). */
+    if (m->count == 0)
         m->ema = CdCorr;
     else
         m->ema = 0.1f * CdCorr + 0.9f * m->ema;

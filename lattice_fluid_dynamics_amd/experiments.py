@@ -77,21 +77,38 @@ def sweep_bodies(seed: int, nx: int, ny: int, nz: int):
     return geometry.random_body(seed)
 
 
+def config4_tunnels(ntun: int = 64, rng_seed: int = 12345):
+    """The tunnels of the batched sweep: nx uniform in [64, 192], ny = nz =
+    nx / 2, inlet u uniform in [0.02, 0.08], tau uniform in [0.51, 0.8], body
+    seed = tunnel index.  A list of dicts (seed, dims, u, tau)."""
+    rng = np.random.default_rng(rng_seed)
+    out = []
+    for s in range(ntun):
+        nx = int(rng.integers(64, 193))
+        u = float(rng.uniform(0.02, 0.08))
+        tau = float(rng.uniform(0.51, 0.8))
+        out.append({"seed": s, "dims": [nx, nx // 2, nx // 2], "u": u, "tau": tau})
+    return out
+
+
+def config4_geometry(t):
+    """A grid holding tunnel t's body (solid mask and Bouzidi links), for
+    LBM_BatchAddTunnel or a single run; returns (grid, body kind)."""
+    nx, ny, nz = t["dims"]
+    g = lbm.LBM_Create(nx, ny, nz, (t["tau"] - 0.5) / 3)
+    tris, bbox, kind = sweep_bodies(t["seed"], nx, ny, nz)
+    lbm.LBM_SetSolidMesh(g, tris, bbox)
+    return g, kind
+
+
 def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 20):
-    rng = np.random.default_rng(12345)
     b = lbm.Batch(storage=storage, useMRT=1, useSmagorinsky=0, periodicYZ=1)
     meta = []
     t0 = time.time()
-    for s in range(ntun):
-        nx = int(rng.integers(64, 193))
-        ny = nz = nx // 2
-        u = float(rng.uniform(0.02, 0.08))
-        tau = float(rng.uniform(0.51, 0.8))
-        g = lbm.LBM_Create(nx, ny, nz, (tau - 0.5) / 3)
-        tris, bbox, kind = sweep_bodies(s, nx, ny, nz)
-        lbm.LBM_SetSolidMesh(g, tris, bbox)
-        b.add(g, tau, u)
-        meta.append({"seed": s, "dims": [nx, ny, nz], "u": u, "tau": tau, "body": kind})
+    for t in config4_tunnels(ntun):
+        g, kind = config4_geometry(t)
+        b.add(g, t["tau"], t["u"])
+        meta.append({**t, "body": kind})
         g.free()
     b.initialize()
     setup_s = time.time() - t0

@@ -86,6 +86,7 @@ SIGNATURES = {
     "LBM_Synchronize": (None, [GridP]),
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
+    "LBM_GetGuardResetCount": (ctypes.c_longlong, [GridP]),
     "LBM_GetLastError": (ctypes.c_char_p, []),
     "LBM_CdMonitorCreate": (c_void_p, [GridP, c_float, c_float, c_int]),
     "LBM_CdMonitorFree": (None, [c_void_p]),
@@ -114,6 +115,7 @@ SIGNATURES = {
     "LBM_BatchSetBlockOrder": (c_int, [c_void_p, c_int]),
     "LBM_BatchGetKernelConfig": (c_int, [c_void_p, c_int_p, c_int_p, c_int_p]),
     "LBM_BatchTunnelCount": (c_int, [c_void_p]),
+    "LBM_BatchGuardResetCount": (ctypes.c_longlong, [c_void_p]),
     "LBM_BatchTotalCells": (ctypes.c_longlong, [c_void_p]),
     "LBM_BatchComputeForcesF64": (c_int, [c_void_p, c_double_p, ctypes.POINTER(ctypes.c_longlong)]),
     "LBM_BatchComputeCoefficients": (c_int, [c_void_p, c_float_p]),
@@ -339,6 +341,13 @@ def LBM_GetBoundaryCellCount(g: Grid) -> int:
     return int(load_library().LBM_GetBoundaryCellCount(g.ptr))
 
 
+def LBM_GetGuardResetCount(g: Grid) -> int:
+    n = int(load_library().LBM_GetGuardResetCount(g.ptr))
+    if n < 0:
+        raise RuntimeError("LBM_GetGuardResetCount failed: " + last_error())
+    return n
+
+
 class DeviceParticles:
     """Device array of n 32-byte Particle records."""
 
@@ -534,6 +543,13 @@ class Batch:
     @property
     def ntunnels(self) -> int:
         return int(load_library().LBM_BatchTunnelCount(self.ptr))
+
+    @property
+    def guard_resets(self) -> int:
+        n = int(load_library().LBM_BatchGuardResetCount(self.ptr))
+        if n < 0:
+            raise RuntimeError("LBM_BatchGuardResetCount failed: " + last_error())
+        return n
 
     @property
     def total_cells(self) -> int:

@@ -107,3 +107,56 @@ def test_grid_beyond_2_30_cells(gpu):
     large = run(1024, 1025, True)
     print("mass per cell: 1024x4x5 %.15f, 1024x1024x1025 %.15f" % (small, large))
     assert abs(large - small) <= 1e-10 and abs(large - 1.0) < 1e-4
+
+
+def test_config3_runs_its_10000_steps(gpu):
+    """The roofline configuration for its configured 10,000 steps, as the
+    reference's main loop drives it: 10 LBM steps per frame, then the
+    50,000 particles (seeded on the inlet plane from seed 0) advected by RK2
+    trilinear sampling and respawned when they leave the box.
+
+    At tau 0.505 the reference scheme does not stay stable at its Zou-He
+    pressure outlet: from step ~500 the outlet plane develops |u| > 0.2, by
+    step ~2500 the stability guard starts resetting outlet cells, and the
+    total mass grows by ~11 % through the open boundaries.  FP64 storage
+    behaves the same (1.0 M resets and +11.7 % mass against 0.70 M and
+    +11.4 % in FP32), so this is the scheme, not rounding.  The test checks
+    what must still hold: finite fields and particles, every cell with
+    |u| > 0.2 within 8 cells of the outlet, guard resets on fewer than 1e-5
+    of the cell updates, the mass within +-20 %, all particles inside the
+    world box.  Cd, the guard count and the mass are printed."""
+    g = lbm.LBM_Create(NX, NY, NZ, (TAU - 0.5) / 3)
+    configure(g)
+    tris, bbox = geometry.car_proxy(seed=0)
+    lbm.LBM_SetSolidMesh(g, tris, bbox)
+    A = lbm.LBM_ComputeProjectedArea(g, 0)
+    assert lbm.LBM_InitializeFlow(g, U, 0, 0) == 1
+    m0 = lbm.LBM_ComputeTotalMass(g)
+    parts = lbm.DeviceParticles(50000)
+    lbm.LBM_SeedParticles(g, parts, 0)
+    cds = []
+    for frame in range(1000):
+        lbm.LBM_StepN(g, 10, U, 0, 0)
+        lbm.LBM_AdvectParticles(g, parts, 10.0, 8.0 / NX, 0, frame + 1)
+        if (frame + 1) % 100 == 0:
+            fx, fy, fz = lbm.LBM_ComputeDragForce(g)
+            cds.append(abs(fx) / (0.5 * U * U * A))
+    vel = lbm.LBM_CopyVelocityToHost(g)
+    m = lbm.LBM_ComputeTotalMass(g)
+    resets = lbm.LBM_GetGuardResetCount(g)
+    p = parts.download()
+    speed = np.sqrt(np.sum(vel[:, :3].astype(np.float64) ** 2, axis=1))
+    hot_x = np.nonzero(speed > 0.2)[0] % NX
+    print("512x256x256, 10000 steps: Cd every 1000 steps %s, mass ratio %.6f, guard resets %d (%.2e of cell "
+          "updates), max |u| %.4f, cells with |u| > 0.2: %d at x in [%s, %s]" % (
+              np.round(cds, 4), m / m0, resets, resets / (NX * NY * NZ * 1e4), speed.max(), len(hot_x),
+              hot_x.min() if len(hot_x) else "-", hot_x.max() if len(hot_x) else "-"))
+    assert g.stepCount == 10000
+    assert np.all(np.isfinite(vel)) and np.all(np.isfinite(cds))
+    assert len(hot_x) == 0 or hot_x.min() >= NX - 8
+    assert resets < 1e-5 * NX * NY * NZ * 1e4
+    assert abs(m / m0 - 1) < 0.2
+    assert np.all(np.isfinite(p))
+    assert np.all((p[:, 0] >= -4) & (p[:, 0] <= 4) & (np.abs(p[:, 1]) <= 2) & (np.abs(p[:, 2]) <= 2))
+    parts.free()
+    lbm.LBM_Free(g)

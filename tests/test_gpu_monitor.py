@@ -46,6 +46,32 @@ def test_monitor_algebra_and_ema(gpu):
         assert s["ema"] == pytest.approx(ema, rel=1e-5)
 
 
+def test_monitor_ema_restarts_until_a_cd_enters_the_history(gpu):
+    """main.c:1309-1314 restarts the EMA from the sample itself while the
+    convergence history is empty, and only samples with 0 < Cd < 1000 enter
+    it.  A reference speed of 1e-3 (50x below the flow's) puts Cd above 1000:
+    every sample restarts the EMA.  With the speed of the flow, the samples
+    enter the history and the EMA averages from the second sample on."""
+    g = sphere_grid()
+    A = lbm.LBM_ComputeProjectedArea(g, 0)
+    slow = lbm.CdMonitor(g, 1e-3, A)
+    prev = None
+    for k in range(4):
+        lbm.LBM_StepN(g, 50, U, 0, 0)
+        _, s = slow.sample()
+        assert s["Cd"] >= 1000
+        assert s["ema"] == s["Cd_corr"]
+        assert prev is None or s["Cd_corr"] != prev
+        prev = s["Cd_corr"]
+    mon = lbm.CdMonitor(g, U, A)
+    lbm.LBM_StepN(g, 50, U, 0, 0)
+    _, s1 = mon.sample()
+    lbm.LBM_StepN(g, 50, U, 0, 0)
+    _, s2 = mon.sample()
+    assert 0 < s1["Cd"] < 1000 and s1["ema"] == s1["Cd_corr"]
+    assert s2["ema"] == pytest.approx(0.1 * s2["Cd_corr"] + 0.9 * s1["ema"], rel=1e-6)
+
+
 def test_monitor_converges_on_a_settled_flow(gpu):
     """The convergence flag fires the first time the relative standard
     deviation of the last 100 Cd samples drops below 1 %, and reports that
