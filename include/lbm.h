@@ -88,6 +88,19 @@ int LBM_InitializeFlow(LBMGrid *grid, float ux, float uy, float uz);
  * field.  The inlet velocity passed to LBM_Step is unaffected. */
 int LBM_InitializeFlowFromField(LBMGrid *grid, const float *field, int cnx, int cny, int cnz);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops that re-initialise tunnels from predicted fields,
+ * harnesses that forbid hipMalloc): LBM_InitializeFlowFromField with the
+ * same arguments, results and errors, except that it calls no device
+ * allocator -- the coarse field is staged in a static per-GPU array of
+ * 4 MiB, so fields above 262,144 coarse cells are rejected (0, with an
+ * error).  It also refuses (0) what would allocate: a populationStorage
+ * different from the grid's current device format, or geometry / periodicYZ
+ * changed since the last LBM_InitializeFlow*; initialise once with an
+ * allocating call first.  Calls on one GPU share the staging array: calls
+ * that overlap on one GPU (grids driven from several threads) race. */
+int LBM_InitializeFlowFromFieldNoAlloc(LBMGrid *grid, const float *field, int cnx, int cny, int cnz);
+
 /* simulation/lib/lbm.h:77  one collide + stream step with the given inlet velocity. */
 void LBM_Step(LBMGrid *grid, float inletVelX, float inletVelY, float inletVelZ);
 

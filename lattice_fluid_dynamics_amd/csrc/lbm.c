@@ -629,6 +629,36 @@ int LBM_InitializeFlowFromField(LBMGrid *g, const float *field, int cnx, int cny
     return 1;
 }
 
+int LBM_InitializeFlowFromFieldNoAlloc(LBMGrid *g, const float *field, int cnx, int cny, int cnz) {
+    if (!g) return 0;
+    if (!field || cnx <= 0 || cny <= 0 || cnz <= 0 || cnx > g->sizeX || cny > g->sizeY || cnz > g->sizeZ) {
+        note_error("LBM_InitializeFlowFromFieldNoAlloc: the field must be a non-empty lattice no finer than the grid");
+        return 0;
+    }
+    Impl *im = impl_of(g);
+    if (!live_dev(g)) return 0;
+    /* Both of these would allocate device memory (new population buffers, a
+     * larger boundary list): the allocating entry points do them. */
+    if (g->populationStorage != lbmdev_storage(im->dev)) {
+        note_error("LBM_InitializeFlowFromFieldNoAlloc: switching the storage format allocates; initialise once "
+                   "with LBM_InitializeFlow or LBM_InitializeFlowFromField first");
+        return 0;
+    }
+    if (im->geomDirty || im->builtPeriodic != (g->periodicYZ == 1)) {
+        note_error("LBM_InitializeFlowFromFieldNoAlloc: the geometry or y/z boundary mode changed since the last "
+                   "upload, which may allocate; initialise once with LBM_InitializeFlow or "
+                   "LBM_InitializeFlowFromField first");
+        return 0;
+    }
+    if (!lbmdev_init_field_noalloc(im->dev, field, cnx, cny, cnz)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    im->flowReady = 1;
+    g->stepCount = 0;
+    return 1;
+}
+
 static void step_params(const LBMGrid *g, float ux, float uy, float uz, LbmStepParams *p) {
     p->tau = g->tau;
     p->smagorinskyCs = g->smagorinskyCs;

@@ -49,6 +49,13 @@ int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, const int32_
 int lbmdev_init_flow(LbmDev *d, const float feq[19], const float vel4[4]);
 /* equilibrium of a coarse (ux, uy, uz, rho) host field, trilinear upsampling */
 int lbmdev_init_field(LbmDev *d, const float *coarse_host, int cnx, int cny, int cnz);
+/* Bytes of the static staging array of lbmdev_init_field_noalloc: 4 MiB,
+ * 262,144 coarse cells of (ux, uy, uz, rho). */
+#define LBMDEV_NOALLOC_FIELD_BYTES (4 << 20)
+/* As lbmdev_init_field without any device allocation: the coarse field is
+ * staged in a static per-GPU array; fields above LBMDEV_NOALLOC_FIELD_BYTES
+ * are rejected.  Calls on one GPU share the array and must not overlap. */
+int lbmdev_init_field_noalloc(LbmDev *d, const float *coarse_host, int cnx, int cny, int cnz);
 
 /* n LBM steps; the velocity buffer is written on the last one. */
 int lbmdev_step(LbmDev *d, const LbmStepParams *p, int nsteps);

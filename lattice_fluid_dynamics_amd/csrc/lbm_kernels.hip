@@ -34,6 +34,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "lbm_device.h"
 
 namespace {
@@ -1275,6 +1277,7 @@ struct LbmDev {
     long long *resi;
     unsigned long long *guard;  // stability-guard resets since the flow was initialised
     float *feq_d;
+    void *scratch;  // this device's address of the static staging array `scratch`
     int nparts;  // partial slots allocated
     int bs;      // workgroup size of the step kernel
     int nt;      // LBMDEV_NT_A / LBMDEV_NT_B: non-temporal population access per phase
@@ -1282,6 +1285,37 @@ struct LbmDev {
 };
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
+
+// ------------------------------------------------ staging without allocation
+// A file-scope device array: the staging buffer of lbmdev_init_field_noalloc,
+// which calls no allocator.  Each GPU the library runs on holds its own copy
+// for the life of the process.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address on each GPU is taken once
+// with hipGetSymbolAddress (with that GPU current) and only that address is
+// used, as kernel argument and copy target.  Every call on one GPU shares the
+// array, so such calls must not overlap on a GPU (they race).
+static __device__ __align__(256) unsigned char scratch[LBMDEV_NOALLOC_FIELD_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_addr[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+// The current device's address of `scratch`, resolved once per device.
+static hipError_t scratch_address(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = NULL;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return hipSuccess;
+}
 
 static size_t elem_size(int fmt) {
     switch (fmt) {
@@ -1345,6 +1379,11 @@ extern "C" LbmDev *lbmdev_create(int nx, int ny, int nz, int storage, char *err,
 #undef ALLOC
     if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) {
         snprintf(err, errlen, "stream creation failed: %s", hipGetErrorString(e));
+        lbmdev_destroy(d);
+        return NULL;
+    }
+    if ((e = scratch_address(&d->scratch)) != hipSuccess) {
+        snprintf(err, errlen, "address of the staging array: %s", hipGetErrorString(e));
         lbmdev_destroy(d);
         return NULL;
     }
@@ -1481,6 +1520,31 @@ static void init_field_dispatch(int fmt, void *pop, const Geo &g, float4 *vel, c
 
 // Equilibrium initial state from a coarse (ux, uy, uz, rho) field on the host
 // (cnz x cny x cnx x 4 floats), upsampled on the device.
+// As lbmdev_init_field, staging the coarse field in the static array
+// `scratch` (through the address resolved when the grid was created) instead
+// of an allocation.  Rejects fields larger than the array.
+extern "C" int lbmdev_init_field_noalloc(LbmDev *d, const float *coarse_host, int cnx, int cny, int cnz) {
+    const size_t bytes = sizeof(float) * 4 * (size_t)cnx * cny * cnz;
+    if (bytes > (size_t)LBMDEV_NOALLOC_FIELD_BYTES) {
+        snprintf(g_err, sizeof(g_err),
+                 "coarse field of %dx%dx%d cells (%zu bytes) exceeds the %d-byte staging array (%d cells)", cnx, cny,
+                 cnz, bytes, LBMDEV_NOALLOC_FIELD_BYTES, LBMDEV_NOALLOC_FIELD_BYTES / 16);
+        return 0;
+    }
+    if (!d->scratch) {
+        snprintf(g_err, sizeof(g_err), "the staging array has no device address");
+        return 0;
+    }
+    HIPCHECK(hipMemcpyAsync(d->scratch, coarse_host, bytes, hipMemcpyHostToDevice, d->stream), "field staging");
+    HIPCHECK(hipMemsetAsync(d->guard, 0, 8, d->stream), "guard counter");
+    init_field_dispatch(d->fmt, d->pop, make_geo(d->nx, d->ny, d->nz, d->stride), d->vel, (const float *)d->scratch,
+                        cnx, cny, cnz, d->stream);
+    HIPCHECK(hipGetLastError(), "field init");
+    HIPCHECK(hipStreamSynchronize(d->stream), "field init");
+    d->state = ST_INIT;
+    return 1;
+}
+
 extern "C" int lbmdev_init_field(LbmDev *d, const float *coarse_host, int cnx, int cny, int cnz) {
     const size_t bytes = sizeof(float) * 4 * (size_t)cnx * cny * cnz;
     float *c = NULL;

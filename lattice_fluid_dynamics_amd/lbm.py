@@ -58,6 +58,7 @@ SIGNATURES = {
     "LBM_SetSolidAABB": (None, [GridP] + [c_float] * 6),
     "LBM_InitializeFlow": (c_int, [GridP, c_float, c_float, c_float]),
     "LBM_InitializeFlowFromField": (c_int, [GridP, c_float_p, c_int, c_int, c_int]),
+    "LBM_InitializeFlowFromFieldNoAlloc": (c_int, [GridP, c_float_p, c_int, c_int, c_int]),
     "LBM_Step": (None, [GridP, c_float, c_float, c_float]),
     "LBM_GetVelocityBuffer": (c_void_p, [GridP]),
     "LBM_ComputeDragForce": (None, [GridP, c_float_p, c_float_p, c_float_p]),
@@ -232,6 +233,15 @@ def LBM_InitializeFlowFromField(g: Grid, field: np.ndarray) -> int:
     assert f.ndim == 4 and f.shape[3] == 4
     cnz, cny, cnx = f.shape[:3]
     return load_library().LBM_InitializeFlowFromField(g.ptr, _fp(f), cnx, cny, cnz)
+
+
+def LBM_InitializeFlowFromFieldNoAlloc(g: Grid, field: np.ndarray) -> int:
+    """As LBM_InitializeFlowFromField, without device allocation (the field
+    is staged in a static 4 MiB per-GPU array; larger fields are refused)."""
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    assert f.ndim == 4 and f.shape[3] == 4
+    cnz, cny, cnx = f.shape[:3]
+    return load_library().LBM_InitializeFlowFromFieldNoAlloc(g.ptr, _fp(f), cnx, cny, cnz)
 
 
 def LBM_Step(g: Grid, ux: float, uy: float, uz: float) -> None:
