@@ -536,7 +536,7 @@ static int ensure_geometry(LBMGrid *g) {
         return 0;
     }
     lbm_boundary_fill(g, periodic, cells, q, n, 0);
-    int ok = lbmdev_set_geometry(im->dev, im->solid, n, cells, q);
+    int ok = lbmdev_set_geometry(im->dev, im->solid, n, cells, q, periodic);
     free(cells);
     free(q);
     if (!ok) {

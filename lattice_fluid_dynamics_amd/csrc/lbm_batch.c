@@ -165,7 +165,7 @@ int LBM_BatchInitialize(LBMBatch *b) {
             ok = 0;
         }
     }
-    if (ok) ok = lbmbatch_set_geometry(b->dev, solid, nper, lcells, q) && lbmbatch_init_flow(b->dev, feq, vel4);
+    if (ok) ok = lbmbatch_set_geometry(b->dev, solid, nper, lcells, q, b->periodicYZ == 1) && lbmbatch_init_flow(b->dev, feq, vel4);
     if (!ok && b->dev) batch_error(lbmdev_last_error());
     free(dims);
     free(tau);

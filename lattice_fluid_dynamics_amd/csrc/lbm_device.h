@@ -40,10 +40,12 @@ int lbmdev_storage(const LbmDev *d);
 const char *lbmdev_last_error(void);
 
 /* solid: int per cell (0/1/2).  ovr_cells: sorted cell indices of the
- * boundary cells whose incoming populations are computed by the stream-rule
- * prepass; ovr_q: q[i*n_ovr + k] for those cells (-1 = no link). */
+ * boundary cells whose incoming populations the stream-rule prepass computes
+ * (for the directions with a link or a source outside the domain); ovr_q:
+ * q[i*n_ovr + k] for those cells (-1 = no link); periodic: the y/z mode the
+ * list was built for. */
 int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, const int32_t *ovr_cells,
-                        const float *ovr_q);
+                        const float *ovr_q, int periodic);
 
 /* f = f_new = feq (19 host-computed values) everywhere, vel = vel4. */
 int lbmdev_init_flow(LbmDev *d, const float feq[19], const float vel4[4]);
@@ -114,7 +116,7 @@ void lbmbatch_destroy(LbmBatchDev *b);
 /* solid: concatenated per tunnel; n_ovr_per: list length per tunnel; ovr_cells:
  * concatenated local cell indices; ovr_q: q[i*Ntot + k] over the whole list. */
 int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const int *n_ovr_per, const int32_t *ovr_cells,
-                          const float *ovr_q);
+                          const float *ovr_q, int periodic);
 int lbmbatch_init_flow(LbmBatchDev *b, const float *feq, const float *vel4);
 int lbmbatch_init_tunnel_field(LbmBatchDev *b, int t, const float *coarse_host, int cnx, int cny, int cnz);
 int lbmbatch_step(LbmBatchDev *b, const LbmBatchParams *p, int nsteps, int use_graph);

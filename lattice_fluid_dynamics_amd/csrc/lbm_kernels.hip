@@ -520,6 +520,7 @@ struct StepArgs {
     typename Pop<FMT>::S *pop;
     const uint8_t *mask;
     const int32_t *ovr_index;
+    const uint32_t *ovr_bits;  // per listed cell: directions whose value comes from the list
     const typename Pop<FMT>::T *ovr_vals;
     int ovr_stride;
     int use_ovr;
@@ -608,8 +609,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         // moves to the lane that needs it: for ex_i = +1 from lane l - 1, for
         // ex_i = -1 from lane l + 1 (DPP), the wave-edge values through LDS
         // from the neighbouring wave of the row.  Every lane loads, listed
-        // cells included, because its neighbours need the values; listed
-        // cells then take their incoming values from the boundary list.
+        // cells included, because its neighbours need the values.
         typedef typename P::S S;
         __shared__ S lp[8][5], lm[8][5];
         S raw[19];
@@ -634,15 +634,6 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
             }
             f[i] = P::dec(v, OPP[i]);
         }
-        if (ovr) {
-            const int k = a.ovr_index[cell];
-#pragma unroll
-            for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
-        }
-    } else if (ovr) {
-        const int k = a.ovr_index[cell];
-#pragma unroll
-        for (int i = 0; i < 19; i++) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
     } else if (PHASE == 0) {
 #pragma unroll
         for (int i = 0; i < 19; i++) f[i] = P::dec(io.ld(i, (uint32_t)cell), i);
@@ -653,6 +644,17 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
             const uint32_t src = (uint32_t)(X[1 - EX[i]] + Y[1 - EY[i]] + Z[1 - EZ[i]]);
             f[i] = P::dec(io.ld(j, src), j);
         }
+    }
+    // A listed cell's loaded value is the plain pull of the torus streaming,
+    // which is the reference's stream rule for every direction without a
+    // Bouzidi link and with its source inside the domain (or across a
+    // periodic face).  The other directions take the prepass's value.
+    if (ovr) {
+        const int k = a.ovr_index[cell];
+        const uint32_t bits = a.ovr_bits[k];
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+            if (bits & (1u << i)) f[i] = a.ovr_vals[(long long)i * a.ovr_stride + k];
     }
 
     T fo[19];
@@ -669,10 +671,11 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
     if (a.write_vel) a.vel[cell] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
 
     if (PHASE == 0) {
+        const SlotIO<typename P::S, NT ? CP_NT : 0> ios(opaque_ptr(a.pop), g);
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             const int j = OPP[i];
-            io.st(j, (uint32_t)cell, P::enc(fo[i], j));
+            ios.st(j, (uint32_t)cell, P::enc(fo[i], j));
         }
     } else if (!ROW) {
         // Offsets recomputed from an opaque copy of the cell index (see the
@@ -763,6 +766,7 @@ struct BatchArgs {
     typename Pop<FMT>::S *pop;
     const uint8_t *mask;
     const int32_t *ovr_index;
+    const uint32_t *ovr_bits;
     const typename Pop<FMT>::T *ovr_vals;
     int ovr_stride;
     int use_ovr;
@@ -788,6 +792,7 @@ __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     a.pop = b.pop + d.popOff;
     a.mask = b.mask + d.cellOff;
     a.ovr_index = b.ovr_index + d.cellOff;
+    a.ovr_bits = b.ovr_bits;
     a.ovr_vals = b.ovr_vals;
     a.ovr_stride = b.ovr_stride;
     a.use_ovr = b.use_ovr;
@@ -813,14 +818,21 @@ struct BoundArgs {
     int n_ovr;
     int periodic;
     Geo g;
+    // Work list: entries seg[i] .. seg[i+1] - 1 of pairs are the list slots k
+    // whose direction i needs the rule (ovr_bits), each direction's segment
+    // padded with -1 to a multiple of 64.
+    const int32_t *pairs;
+    int seg[20];
 };
 
-// Boundary prepass: the incoming populations of every listed cell under the
-// reference stream rule, evaluated on the stored post-collision field.  One
-// lane per (listed cell, direction): the list is padded to a multiple of 64
-// per direction, so each wavefront evaluates a single direction (made
-// explicit with readfirstlane and dispatched to a constant-direction copy of
-// the rule), and stores to ovr_vals[i * n + k] are contiguous.
+// Boundary prepass: the incoming populations of the listed cells under the
+// reference stream rule, evaluated on the stored post-collision field, for
+// the (cell, direction) pairs whose value the plain torus pull of the main
+// kernel does not give (a Bouzidi link, a source across an x face or a
+// clamped y/z face).  One lane per pair; the pairs are grouped by direction
+// and each group padded to a multiple of 64, so each wavefront evaluates a
+// single direction (found from its uniform first index and dispatched to a
+// constant-direction copy of the rule).
 #define LBM_PREPASS_CASE(I) \
     case I: v = stream_value<FMT, STATE>(pop, g, periodic, x, y, z, I, qo); break;
 
@@ -839,16 +851,68 @@ __device__ __forceinline__ typename Pop<FMT>::T prepass_value(const typename Pop
 }
 #undef LBM_PREPASS_CASE
 
-__host__ __device__ inline int prepass_pad(int n) { return (n + 63) & ~63; }
+// The directions of listed cell k (slot k of a [19][qstride] q table) whose
+// incoming value the plain torus pull does not give: a Bouzidi link (q of the
+// opposite direction set), a source across an x face, or across a clamped
+// y/z face.
+static uint32_t special_dirs(int nx, int ny, int nz, int periodic, int cell, const float *q, long long qstride,
+                             long long k) {
+    const int x = cell % nx, t = cell / nx, y = t % ny, z = t / ny;
+    uint32_t bits = 0;
+    for (int i = 0; i < 19; i++) {
+        const int X = x - EX[i], Y = y - EY[i], Z = z - EZ[i];
+        const bool link = q[(long long)OPP[i] * qstride + k] >= 0.0f;
+        const bool xout = X < 0 || X >= nx;
+        const bool yzout = !periodic && (Y < 0 || Y >= ny || Z < 0 || Z >= nz);
+        if (link || xout || yzout) bits |= 1u << i;
+    }
+    return bits;
+}
+
+// The prepass work list of n listed cells with direction sets bits[k]: seg
+// and a malloc'd pairs array (NULL with seg[19] = 0 when empty or on
+// allocation failure; returns 0 on failure).
+static int build_pairs(int n, const uint32_t *bits, int32_t **pairs, int seg[20]) {
+    int total = 0;
+    for (int i = 0; i < 19; i++) {
+        int c = 0;
+        for (int k = 0; k < n; k++) c += (bits[k] >> i) & 1;
+        seg[i] = total;
+        total += (c + 63) & ~63;
+    }
+    seg[19] = total;
+    *pairs = NULL;
+    if (total == 0) return 1;
+    int32_t *p = (int32_t *)malloc(sizeof(int32_t) * (size_t)total);
+    if (!p) return 0;
+    for (int i = 0; i < 19; i++) {
+        int w = seg[i];
+        for (int k = 0; k < n; k++)
+            if ((bits[k] >> i) & 1) p[w++] = k;
+        while (w < seg[i + 1]) p[w++] = -1;
+    }
+    *pairs = p;
+    return 1;
+}
+
+// The direction of the wavefront holding list entry idx (uniform: segments
+// and wavefronts are 64-aligned), or 19 past the last segment.
+template <int FMT>
+__device__ __forceinline__ int prepass_dir(const BoundArgs<FMT> &a, int idx) {
+    const int wbase = __builtin_amdgcn_readfirstlane(idx & ~63);
+    int i = 0;
+#pragma unroll
+    for (int j = 1; j < 20; j++) i += wbase >= a.seg[j];
+    return i;
+}
 
 template <int FMT, int STATE>
 __global__ __launch_bounds__(BLOCK) void k_prepass(BoundArgs<FMT> a) {
-    if (a.n_ovr <= 0) return;
-    const int npad = prepass_pad(a.n_ovr);
     const int idx = blockIdx.x * BLOCK + threadIdx.x;
-    const int i = __builtin_amdgcn_readfirstlane(idx / npad);  // uniform: npad and waves are 64-aligned
-    const int k = idx - i * npad;
-    if (i >= 19 || k >= a.n_ovr) return;
+    const int i = prepass_dir(a, idx);
+    if (i >= 19) return;
+    const int k = a.pairs[idx];
+    if (k < 0) return;
     int x, y, z;
     decompose(a.g, a.ovr_cells[k], x, y, z);
     const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
@@ -859,12 +923,11 @@ __global__ __launch_bounds__(BLOCK) void k_prepass(BoundArgs<FMT> a) {
 template <int FMT, int STATE>
 __global__ __launch_bounds__(BLOCK) void k_prepass_batch(BoundArgs<FMT> a, const int32_t *ovr_tunnel,
                                                          const TunnelDesc *desc) {
-    if (a.n_ovr <= 0) return;
-    const int npad = prepass_pad(a.n_ovr);
     const int idx = blockIdx.x * BLOCK + threadIdx.x;
-    const int i = __builtin_amdgcn_readfirstlane(idx / npad);
-    const int k = idx - i * npad;
-    if (i >= 19 || k >= a.n_ovr) return;
+    const int i = prepass_dir(a, idx);
+    if (i >= 19) return;
+    const int k = a.pairs[idx];
+    if (k < 0) return;
     const TunnelDesc d = desc[ovr_tunnel[k]];
     int x, y, z;
     decompose(d.g, a.ovr_cells[k], x, y, z);
@@ -1269,6 +1332,10 @@ struct LbmDev {
     int32_t *ovr_cells;
     float *ovr_q;
     void *ovr_vals;
+    uint32_t *ovr_bits;  // per listed cell: directions taken from ovr_vals
+    int32_t *ovr_pairs;  // prepass work list (see BoundArgs)
+    int pair_cap;
+    int seg[20];
     int state;  // -1 uninitialised, ST_INIT / ST_A / ST_B
     hipStream_t stream;
     double *pd;
@@ -1410,6 +1477,8 @@ extern "C" void lbmdev_destroy(LbmDev *d) {
     hipFree(d->ovr_cells);
     hipFree(d->ovr_q);
     hipFree(d->ovr_vals);
+    hipFree(d->ovr_bits);
+    hipFree(d->ovr_pairs);
     hipFree(d->pd);
     hipFree(d->pi);
     hipFree(d->resd);
@@ -1423,14 +1492,20 @@ extern "C" void lbmdev_destroy(LbmDev *d) {
 extern "C" int lbmdev_storage(const LbmDev *d) { return d->fmt; }
 
 extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, const int32_t *ovr_cells,
-                                   const float *ovr_q) {
+                                   const float *ovr_q, int periodic) {
     const int nc = d->ncell;
     HIPCHECK(hipStreamSynchronize(d->stream), "synchronize");
     uint8_t *mask = (uint8_t *)malloc((size_t)nc);
     int32_t *index = (int32_t *)malloc((size_t)nc * 4);
-    if (!mask || !index) {
+    uint32_t *bits = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n_ovr > 0 ? n_ovr : 1));
+    int32_t *pairs = NULL;
+    int seg[20];
+    if (mask && index && bits)
+        for (int k = 0; k < n_ovr; k++) bits[k] = special_dirs(d->nx, d->ny, d->nz, periodic, ovr_cells[k], ovr_q, n_ovr, k);
+    if (!mask || !index || !bits || !build_pairs(n_ovr, bits, &pairs, seg)) {
         free(mask);
         free(index);
+        free(bits);
         snprintf(g_err, sizeof(g_err), "host allocation failed");
         return 0;
     }
@@ -1456,11 +1531,40 @@ extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, c
             hipMalloc((void **)&d->ovr_vals, (size_t)cap * 19 * d->telem) != hipSuccess) {
             free(mask);
             free(index);
+            free(bits);
+            free(pairs);
             snprintf(g_err, sizeof(g_err), "device allocation of the boundary list failed");
             return 0;
         }
         d->ovr_cap = cap;
+        hipFree(d->ovr_bits);
+        d->ovr_bits = NULL;
+        if (hipMalloc((void **)&d->ovr_bits, (size_t)cap * 4) != hipSuccess) {
+            free(mask);
+            free(index);
+            free(bits);
+            free(pairs);
+            d->ovr_cap = 0;
+            snprintf(g_err, sizeof(g_err), "device allocation of the boundary list failed");
+            return 0;
+        }
     }
+    if (seg[19] > d->pair_cap) {
+        hipFree(d->ovr_pairs);
+        d->ovr_pairs = NULL;
+        d->pair_cap = 0;
+        const int cap = seg[19] + 19 * 64;
+        if (hipMalloc((void **)&d->ovr_pairs, (size_t)cap * 4) != hipSuccess) {
+            free(mask);
+            free(index);
+            free(bits);
+            free(pairs);
+            snprintf(g_err, sizeof(g_err), "device allocation of the boundary work list failed");
+            return 0;
+        }
+        d->pair_cap = cap;
+    }
+    memcpy(d->seg, seg, sizeof(seg));
     d->n_ovr = n_ovr;
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMemcpy(d->mask, mask, (size_t)nc, hipMemcpyHostToDevice);
@@ -1469,8 +1573,13 @@ extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, c
     if (e == hipSuccess && n_ovr > 0) e = hipMemcpy(d->ovr_cells, ovr_cells, (size_t)n_ovr * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && n_ovr > 0)
         e = hipMemcpy(d->ovr_q, ovr_q, (size_t)n_ovr * 19 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_ovr > 0) e = hipMemcpy(d->ovr_bits, bits, (size_t)n_ovr * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && seg[19] > 0)
+        e = hipMemcpy(d->ovr_pairs, pairs, (size_t)seg[19] * 4, hipMemcpyHostToDevice);
     free(mask);
     free(index);
+    free(bits);
+    free(pairs);
     if (e != hipSuccess) return set_err("geometry upload", e);
     return 1;
 }
@@ -1575,7 +1684,10 @@ static void launch_prepass(LbmDev *d, int periodic) {
     a.n_ovr = d->n_ovr;
     a.periodic = periodic;
     a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
-    const int nb = (19 * prepass_pad(d->n_ovr) + BLOCK - 1) / BLOCK;
+    a.pairs = d->ovr_pairs;
+    memcpy(a.seg, d->seg, sizeof(a.seg));
+    if (a.seg[19] == 0) return;
+    const int nb = (a.seg[19] + BLOCK - 1) / BLOCK;
     if (d->state == ST_A)
         hipLaunchKernelGGL((k_prepass<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, d->stream, a);
     else
@@ -1617,6 +1729,7 @@ static void launch_main(LbmDev *d, const LbmStepParams *p, int phase, int use_ov
     a.pop = (typename Pop<FMT>::S *)d->pop;
     a.mask = d->mask;
     a.ovr_index = d->ovr_index;
+    a.ovr_bits = d->ovr_bits;
     a.ovr_vals = (const typename Pop<FMT>::T *)d->ovr_vals;
     a.ovr_stride = d->n_ovr;
     a.use_ovr = use_ovr && d->n_ovr > 0;
@@ -1999,6 +2112,9 @@ struct LbmBatchDev {
     int32_t *ovr_cells, *ovr_tunnel;
     float *ovr_q;
     void *ovr_vals;
+    uint32_t *ovr_bits;  // per listed cell: directions taken from ovr_vals
+    int32_t *ovr_pairs;  // prepass work list (see BoundArgs)
+    int seg[20];
     int32_t *block_tunnel;
     int nblocks, bs;
     int smode;  // LBMDEV_NT_A / LBMDEV_NT_B streaming mode of the step kernel
@@ -2144,6 +2260,8 @@ extern "C" void lbmbatch_destroy(LbmBatchDev *b) {
     hipFree(b->ovr_tunnel);
     hipFree(b->ovr_q);
     hipFree(b->ovr_vals);
+    hipFree(b->ovr_bits);
+    hipFree(b->ovr_pairs);
     hipFree(b->block_tunnel);
     hipFree(b->pd);
     hipFree(b->pi);
@@ -2160,7 +2278,7 @@ extern "C" void lbmbatch_destroy(LbmBatchDev *b) {
 // solid: concatenated per tunnel; n_ovr_per[t] list entries of tunnel t;
 // ovr_cells: concatenated local cell indices; ovr_q: planes q[i*Ntot + k].
 extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const int *n_ovr_per,
-                                     const int32_t *ovr_cells, const float *ovr_q) {
+                                     const int32_t *ovr_cells, const float *ovr_q, int periodic) {
     HIPCHECK(hipStreamSynchronize(b->stream), "synchronize");
     const long long nc = b->total_cells;
     int ntot = 0;
@@ -2172,10 +2290,19 @@ extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const
     uint8_t *mask = (uint8_t *)malloc((size_t)nc);
     int32_t *index = (int32_t *)malloc((size_t)nc * 4);
     int32_t *tun = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ntot > 0 ? ntot : 1));
-    if (!mask || !index || !tun) {
+    uint32_t *bits = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(ntot > 0 ? ntot : 1));
+    int32_t *pairs = NULL;
+    int seg[20];
+    if (bits)
+        for (int t = 0; t < b->nt; t++)
+            for (int k = b->kstart[t]; k < b->kstart[t + 1]; k++)
+                bits[k] = special_dirs(b->desc_h[t].g.nx, b->desc_h[t].g.ny, b->desc_h[t].g.nz, periodic, ovr_cells[k],
+                                       ovr_q, ntot, k);
+    if (!mask || !index || !tun || !bits || !build_pairs(ntot, bits, &pairs, seg)) {
         free(mask);
         free(index);
         free(tun);
+        free(bits);
         snprintf(g_err, sizeof(g_err), "host allocation failed");
         return 0;
     }
@@ -2194,11 +2321,17 @@ extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const
     hipFree(b->ovr_tunnel);
     hipFree(b->ovr_q);
     hipFree(b->ovr_vals);
+    hipFree(b->ovr_bits);
+    hipFree(b->ovr_pairs);
     b->ovr_cells = b->ovr_tunnel = NULL;
     b->ovr_q = NULL;
     b->ovr_vals = NULL;
+    b->ovr_bits = NULL;
+    b->ovr_pairs = NULL;
     hipError_t e = hipSuccess;
     const size_t n1 = (size_t)(ntot > 0 ? ntot : 1);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_bits, n1 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_pairs, (size_t)(seg[19] > 0 ? seg[19] : 1) * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_cells, n1 * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_tunnel, n1 * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_q, n1 * 19 * 4);
@@ -2209,10 +2342,16 @@ extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const
     if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_tunnel, tun, (size_t)ntot * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && ntot > 0)
         e = hipMemcpy(b->ovr_q, ovr_q, (size_t)ntot * 19 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_bits, bits, (size_t)ntot * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && seg[19] > 0)
+        e = hipMemcpy(b->ovr_pairs, pairs, (size_t)seg[19] * 4, hipMemcpyHostToDevice);
     free(mask);
     free(index);
     free(tun);
+    free(bits);
+    free(pairs);
     if (e != hipSuccess) return set_err("batch geometry upload", e);
+    memcpy(b->seg, seg, sizeof(seg));
     b->n_ovr = ntot;
     batch_free_graphs(b);
     return 1;
@@ -2282,6 +2421,7 @@ static void batch_launch_main(LbmBatchDev *b, const LbmBatchParams *p, int phase
     a.mask = b->mask;
     a.ovr_index = b->ovr_index;
     a.ovr_vals = (const typename Pop<FMT>::T *)b->ovr_vals;
+    a.ovr_bits = b->ovr_bits;
     a.ovr_stride = b->n_ovr;
     a.use_ovr = use_ovr && b->n_ovr > 0;
     a.vel = b->vel;
@@ -2332,7 +2472,7 @@ static void batch_step_once(LbmBatchDev *b, const LbmBatchParams *p, int wv) {
         phase = 0;
         use_ovr = 0;
     } else {
-        if (b->n_ovr > 0) {
+        if (b->n_ovr > 0 && b->seg[19] > 0) {
             BoundArgs<FMT> a;
             a.pop = (const typename Pop<FMT>::S *)b->pop;
             a.ovr_cells = b->ovr_cells;
@@ -2341,7 +2481,9 @@ static void batch_step_once(LbmBatchDev *b, const LbmBatchParams *p, int wv) {
             a.n_ovr = b->n_ovr;
             a.periodic = p->periodicYZ == 1;
             a.g = make_geo(2, 1, 1, 64);
-            const int nb = (19 * prepass_pad(b->n_ovr) + BLOCK - 1) / BLOCK;
+            a.pairs = b->ovr_pairs;
+            memcpy(a.seg, b->seg, sizeof(a.seg));
+            const int nb = (a.seg[19] + BLOCK - 1) / BLOCK;
             if (b->state == ST_A)
                 hipLaunchKernelGGL((k_prepass_batch<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, b->stream, a,
                                    (const int32_t *)b->ovr_tunnel, (const TunnelDesc *)b->desc_d);
