@@ -668,7 +668,14 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         collide_fluid<OP, T>(f, fo, a.cp, x, g.nx, rho, ux, uy, uz);
     }
 
-    if (a.write_vel) a.vel[cell] = make_float4((float)ux, (float)uy, (float)uz, (float)rho);
+    if (a.write_vel) {
+        // a non-temporal store: the 16 B/cell velocity output streams past the
+        // caches instead of displacing the populations' lines (0.5 % per step
+        // on 512x256x256 writing it every 10th step)
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const v4f v = {(float)ux, (float)uy, (float)uz, (float)rho};
+        __builtin_nontemporal_store(v, (v4f *)&a.vel[cell]);
+    }
 
     if (PHASE == 0) {
         const SlotIO<typename P::S, NT ? CP_NT : 0> ios(opaque_ptr(a.pop), g);
