@@ -11,6 +11,15 @@ the reference's main loop).  The same lattice with regularized collision +
 Smagorinsky (the 512x256x256 roofline configuration of the workload list) is
 measured after it and reported under "secondary".
 
+`time_to_converged_cd` is the metric's second half, measured end to end
+after the throughput runs (bounded to ~15 s): the FP32 solver generates
+training fields for 64 seeded bodies, the warm-start network is trained on
+them by the project's kernels, and on 8 held-out bodies the seconds until Cd
+settles within 1 % of the FP64 cold start are compared for the learned warm
+start with FP16 storage and the plain FP32 cold start.  By default the
+tunnels are 128x64x64 (the learned-start configuration at half resolution);
+--ttc-dims 256x128x128 runs it at full resolution (several minutes).
+
 A "step" is one LBM update of the whole lattice.  `value` is lattice updates
 per second of the whole timed region (inputs resident in HBM; includes the
 boundary prepass, the particle kernel and the host synchronisations).
@@ -182,6 +191,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--secondary", default=SECONDARY, choices=sorted(WORKLOADS) + ["none"],
                     help="a second workload measured after the first and reported under 'secondary'")
+    ap.add_argument("--ttc-dims", default="128x64x64",
+                    help="tunnel size of the time-to-converged-Cd measurement ('none' = skip)")
     ap.add_argument("--event-every", type=int, default=10,
                     help="time the kernel launches of every k-th 10-step frame with HIP events (0 = none)")
     args = ap.parse_args()
@@ -292,7 +303,40 @@ def run(args):
             "avg_launch_ms": r.get("avg_launch_ms"), "achieved_gbs": r.get("achieved"),
             "traffic": r.get("traffic"), "traffic_source": r.get("traffic_source"),
             "workgroup": r.get("workgroup"), "streaming_mode": r.get("streaming_mode"), "cd": s["cd"]}
+    if args.ttc_dims != "none":
+        out["time_to_converged_cd"] = time_to_converged_cd(tuple(int(v) for v in args.ttc_dims.split("x")))
     return out
+
+
+def time_to_converged_cd(dims):
+    """Learned warm start + FP16 storage against the plain FP32 cold start:
+    seconds to Cd within 1 % of the FP64 cold start (experiments.learned_gain)."""
+    from lattice_fluid_dynamics_amd import experiments
+
+    full = dims[0] >= 256
+    kw = dict(data_steps=6000, avg_steps=2000, ref_steps=16000, cold_steps=16000, warm_steps=8000, iters=2000,
+              window=10) if full else {}
+    t0 = time.time()
+    r = experiments.learned_gain(dims=dims, log=log, **kw)
+    return {"metric": "seconds to Cd within 1 % of the FP64 cold start: learned warm start + FP16 storage vs "
+                      "plain FP32 cold start (median over held-out bodies)",
+            "speedup_median": round(r["median_speedup"], 2) if r["median_speedup"] else None,
+            "speedup_min": round(r["min_speedup"], 2) if r["min_speedup"] else None,
+            "speedup_geomean": round(r["geomean_speedup"], 2) if r["geomean_speedup"] else None,
+            "median_steps": r["median_steps"],
+            "median_seconds_per_tunnel": {k: round(float(np.median([row[k + "_s"] for row in r["rows"]
+                                                                    if row[k + "_s"] is not None])), 5)
+                                          for k in ("cold_fp32", "warm_fp16")},
+            "converged": r["converged"], "held_out_tunnels": len(r["rows"]),
+            "ms_per_step_per_tunnel": {k: round(v, 5) for k, v in r["ms_per_step_per_tunnel"].items()},
+            "warm_init_s_per_tunnel": round(r["warm_init_s_per_tunnel"], 5),
+            "training": {"tunnels": r["params"]["n_train"], "data_gen_s": round(r["data_gen_s"], 1),
+                         "adamw_steps": r["params"]["iters"], "train_s": round(r["train_s"], 1),
+                         "loss_first": round(r["loss_first"], 5), "loss_last": round(r["loss_last"], 6)},
+            "workload": f"{dims[0]}x{dims[1]}x{dims[2]} tunnels, coarse lattice 1/4, MRT, periodic y/z; "
+                        "seeded random bodies, u U[0.03,0.07], tau U[0.6,0.9]; Cd every "
+                        f"{r['params']['interval']} steps, window {r['params']['window']}",
+            "wall_s": round(time.time() - t0, 1)}
 
 
 if __name__ == "__main__":

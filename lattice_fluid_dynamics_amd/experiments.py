@@ -136,6 +136,77 @@ def _steps_to_1pct(h, ref, window):
     return None if t is None else int(t)
 
 
+def learned_gain(dims=(128, 64, 64), factor: int = 4, n_train: int = 64, data_steps: int = 4000,
+                 avg_steps: int = 1000, iters: int = 600, batch: int = 16, lr: float = 1e-3, n_eval: int = 8,
+                 ref_steps: int = 8000, cold_steps: int = 8000, warm_steps: int = 4000, interval: int = 100,
+                 window: int = 5, tail: int = 20, log=None):
+    """What the trained warm start gains over the cold start, end to end on
+    the GPU: the FP32 solver generates the training fields (cold starts of
+    the training tunnels, block-averaged to the coarse lattice and averaged
+    over `avg_steps`), the network is trained on them with AdamW by the
+    project's kernels, and on `n_eval` held-out tunnels (seeds 100000..) the
+    time to a converged Cd is measured for the cold FP32 start and for the
+    warm start with FP16 storage.
+
+    Converged Cd: the tunnel's FP64-storage cold start (the GPU path that
+    equals the FP64 oracle to 1e-12) averaged over its last `tail` samples of
+    `ref_steps` steps.  Time to it: the first sample step after which the
+    mean of the last `window` Cd samples (one every `interval` steps) stays
+    within 1 % of it; seconds = steps times the batch's measured time per
+    step divided by the tunnel count, plus the warm start's inference and
+    initialisation time per tunnel.  Returns a dict with per-tunnel rows and
+    the median speed-up (cold FP32 seconds over warm FP16 seconds)."""
+    from . import warmstart as WS
+
+    nx, ny, nz = dims
+    t0 = time.time()
+    specs = WS.random_specs(range(n_train))
+    masks, fields = WS.solve_fields(specs, dims, factor, data_steps, avg_steps, interval, storage=0, chunk=n_train)
+    gen_s = time.time() - t0
+    ws = WS.WarmStart(nx // factor, ny // factor, nz // factor, max_batch=batch, seed=0)
+    data = WS.DeviceDataset(masks, fields, specs, augment=True)
+    t0 = time.time()
+    losses = WS.train(ws, data, iters, batch=batch, lr=lr, wd=1e-4)
+    train_s = time.time() - t0
+    del data
+    if log:
+        log(f"learned gain: {n_train} training tunnels solved in {gen_s:.1f}s, {iters} AdamW steps in "
+            f"{train_s:.1f}s (loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.5f})")
+    ev = WS.random_specs(range(100000, 100000 + n_eval))
+    ref = WS.cd_histories(ev, dims, ref_steps, 3, None, factor, interval)
+    cold = WS.cd_histories(ev, dims, cold_steps, 0, None, factor, interval)
+    warm = WS.cd_histories(ev, dims, warm_steps, 1, ws, factor, interval)
+    ws.free()
+    rows = []
+    for t, sp in enumerate(ev):
+        cd_ref = ref["histories"][t].converged_value(tail=tail)
+        row = {"seed": sp.seed, "u": round(sp.u, 5), "tau": round(sp.tau, 4), "cd_fp64": round(cd_ref, 5)}
+        for name, r in (("cold_fp32", cold), ("warm_fp16", warm)):
+            st = _steps_to_1pct(r["histories"][t], cd_ref, window)
+            row[name + "_steps"] = st
+            row[name + "_s"] = None if st is None else st * r["ms_per_step"] / n_eval / 1e3 + r["init_s"]
+        row["speedup"] = (row["cold_fp32_s"] / row["warm_fp16_s"]
+                          if row["cold_fp32_s"] is not None and row["warm_fp16_s"] is not None else None)
+        rows.append(row)
+    sp = [r["speedup"] for r in rows if r["speedup"] is not None]
+    return {"rows": rows, "median_speedup": float(np.median(sp)) if sp else None,
+            "min_speedup": float(np.min(sp)) if sp else None,
+            "geomean_speedup": float(np.exp(np.mean(np.log(sp)))) if sp else None,
+            "converged": {k: sum(r[k + "_steps"] is not None for r in rows) for k in ("cold_fp32", "warm_fp16")},
+            "median_steps": {k: float(np.median([r[k + "_steps"] for r in rows if r[k + "_steps"] is not None]))
+                             if any(r[k + "_steps"] is not None for r in rows) else None
+                             for k in ("cold_fp32", "warm_fp16")},
+            "ms_per_step_per_tunnel": {"cold_fp32": cold["ms_per_step"] / n_eval,
+                                       "warm_fp16": warm["ms_per_step"] / n_eval,
+                                       "ref_fp64": ref["ms_per_step"] / n_eval},
+            "warm_init_s_per_tunnel": warm["init_s"], "data_gen_s": gen_s, "train_s": train_s,
+            "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-20:])),
+            "params": {"dims": list(dims), "factor": factor, "n_train": n_train, "data_steps": data_steps,
+                       "avg_steps": avg_steps, "iters": iters, "batch": batch, "n_eval": n_eval,
+                       "ref_steps": ref_steps, "cold_steps": cold_steps, "warm_steps": warm_steps,
+                       "interval": interval, "window": window, "tail": tail}}
+
+
 def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 6000, avg_steps: int = 2000,
                          cold_steps: int = 20000, warm_steps: int = 12000, batch: int = 16, lr: float = 1e-3,
                          weights_in=None, weights_out=None, chunk: int = 96, window: int = 20,

@@ -22,7 +22,7 @@ def make_geometry(lbm, dims, sphere):
 
 
 @pytest.mark.parametrize("periodic", [1, 0])
-@pytest.mark.parametrize("storage", [0, 1])
+@pytest.mark.parametrize("storage", [0, 1, 3])
 def test_batch_equals_single_grids(gpu, periodic, storage):
     b = gpu.Batch(storage=storage, useMRT=1, useSmagorinsky=1, smagorinskyCs=0.1, periodicYZ=periodic)
     singles = []

@@ -1,0 +1,45 @@
+"""What the trained warm-start network gains over the method without it.
+
+End to end on the GPU, at 128x64x64 tunnels (config 5 at half resolution, so
+that data generation, training and evaluation take seconds):
+
+1. the FP32 solver produces the training data: 64 seeded random bodies
+   (box, ellipsoid or slanted-rear, 10-20 % of the tunnel height), inlet u
+   U[0.03, 0.07], tau U[0.6, 0.9], MRT, periodic y/z, 4000 steps from a
+   cold start, then the 4^3 block-averaged field averaged over 1000 steps;
+2. the 3D conv encoder-decoder is trained on them (and their y/z mirror
+   images) by the project's BF16 MFMA forward/backward kernels and AdamW,
+   600 steps of batch 16;
+3. on 8 held-out bodies (seeds 100000-100007) the time to a converged Cd --
+   the windowed Cd (5 samples, one every 100 steps) staying within 1 % of
+   the tunnel's FP64 cold-start Cd -- is measured for the cold start in
+   FP32 storage and for the warm start (network prediction, trilinear
+   upsampling, equilibrium populations) in FP16 storage.
+
+The FP64 reference is the GPU's FP64-storage path, which equals the FP64
+oracle to rel L2 1e-12.  Every held-out tunnel must converge in both runs,
+and the warm FP16 start must be at least 3x faster in seconds (median over
+the tunnels; each at least 1.5x).  Steps and seconds are printed."""
+import numpy as np
+import pytest
+
+from lattice_fluid_dynamics_amd import experiments
+
+pytestmark = pytest.mark.gpu
+
+
+def test_trained_warm_start_converges_3x_sooner_than_cold_fp32(gpu):
+    r = experiments.learned_gain(log=print)
+    for row in r["rows"]:
+        print("seed %d u %.4f tau %.3f: Cd_fp64 %.4f; cold FP32 %s steps %s s; warm FP16 %s steps %s s; x%s" % (
+            row["seed"], row["u"], row["tau"], row["cd_fp64"], row["cold_fp32_steps"],
+            None if row["cold_fp32_s"] is None else round(row["cold_fp32_s"], 4), row["warm_fp16_steps"],
+            None if row["warm_fp16_s"] is None else round(row["warm_fp16_s"], 4),
+            None if row["speedup"] is None else round(row["speedup"], 2)))
+    print("median speed-up %.2f (min %.2f, geomean %.2f); median steps %s; ms/step/tunnel %s; loss %.4f -> %.5f" % (
+        r["median_speedup"] or 0, r["min_speedup"] or 0, r["geomean_speedup"] or 0, r["median_steps"],
+        {k: round(v, 5) for k, v in r["ms_per_step_per_tunnel"].items()}, r["loss_first"], r["loss_last"]))
+    assert r["loss_last"] < 0.1 * r["loss_first"]
+    assert r["converged"] == {"cold_fp32": 8, "warm_fp16": 8}
+    assert r["median_speedup"] >= 3.0
+    assert r["min_speedup"] >= 1.5
