@@ -181,6 +181,37 @@ void LBM_SeedParticles(LBMGrid *grid, Particle *particles, int n, unsigned int s
 void LBM_AdvectParticles(LBMGrid *grid, Particle *particles, int n, float dt, float velScale,
                          unsigned int seed, unsigned int step);
 
+/* The particle update of the reference's running application, which is
+ * inline GL in main.c:1012-1101 (no C function; simulation/shaders/
+ * particle.comp:218-407): per particle, the velocity of the nearest cell
+ * (grid position clamped to [0.5, N - 1.5]) scaled by 8 is blended with the
+ * wind, flow = mix(wind, 8 u, 0.5); the particle velocity relaxes towards it,
+ * v = mix(v, flow, 0.05), with v.x >= 0.3 wind.x outside the model's box
+ * grown by 0.05; x += v dt; then the collision mode; then the kill box (y, z
+ * clamped to +-1.3; respawn on x = -4 past carMax.x + 2, when slower than
+ * 0.05 beside the model, or when killed by a collision).  padding1 tracks
+ * the flow speed (or Cp with vizMode 8).  Respawn positions and speeds come
+ * from the counter hash of (seed, step, index) instead of the reference's
+ * time-seeded hash.  The uniforms are those main.c:1036-1096 sets. */
+typedef struct {
+    float windX, windY, windZ; /* main.c:1043: (windSpeed * 0.5, 0, 0) */
+    float carMinX, carMinY, carMinZ, carMaxX, carMaxY, carMaxZ, carCenterX, carCenterY, carCenterZ;
+    int collisionMode; /* 0 none, 1 the model's box, 2 its triangles, 3 the grid's body cells */
+    int useLBM;        /* 1: sample the grid's velocity buffer; 0: the wind only */
+    int vizMode;       /* 8: padding1 follows Cp from the density instead of the speed */
+    /* collisionMode 2: device copies of the triangles (12 floats each, as
+     * LBM_SetSolidMesh takes them) and of buildCollisionGrid's arrays
+     * (include/lbm_model.h), its origin, cell size and resolution */
+    const float *triangles;
+    int numTriangles;
+    const int *cellStart, *cellCount, *triIndices;
+    int numIndices;
+    float gridMinX, gridMinY, gridMinZ, gridCellSizeX, gridCellSizeY, gridCellSizeZ;
+    int gridRes;
+} LBMParticleFlow;
+void LBM_AdvectParticlesLive(LBMGrid *grid, Particle *particles, int n, float dt, const LBMParticleFlow *flow,
+                             unsigned int seed, unsigned int step);
+
 void *LBM_DeviceAlloc(unsigned long long bytes);
 void LBM_DeviceFree(void *p);
 int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes);

@@ -7,10 +7,12 @@
  * helpers, so a caller of the reference links against this library instead:
  *   simulation/obj-file-loader/lib/model_loader.h:4-20  (Vertex, Face, Model,
  *                                                        loadOBJ, freeModel)
- *   simulation/lib/model_bounds.h:8-46                   (CarBounds,
+ *   simulation/lib/model_bounds.h:8-54                   (CarBounds,
  *                                                        GPUTriangle,
+ *                                                        CollisionGrid,
  *                                                        computeModelBounds,
- *                                                        createTriangleBuffer)
+ *                                                        createTriangleBuffer,
+ *                                                        buildCollisionGrid)
  *   simulation/lib/voxelize.h:11-62                      (VoxelizeMode,
  *                                                        VoxelGrid,
  *                                                        Voxelize_*)
@@ -70,6 +72,26 @@ CarBounds computeModelBounds(Model *model, float scale, float offsetX, float off
  * skipped); malloc'd, *outCount triangles; NULL for an empty model. */
 GPUTriangle *createTriangleBuffer(Model *model, float scale, float offsetX, float offsetY, float offsetZ,
                                   float rotationY, int *outCount);
+
+#define COLL_GRID_RES 8
+
+/* Uniform R^3 grid (R = COLL_GRID_RES) over the model box grown by 0.05: the
+ * triangles whose bounding box touches each cell, for the per-triangle
+ * particle collisions (LBM_AdvectParticlesLive, collisionMode 2).  Arrays
+ * are malloc'd; the caller frees cellStart, cellCount and triIndices. */
+typedef struct {
+    int *cellStart;  /* offset into triIndices per cell */
+    int *cellCount;  /* triangle count per cell */
+    int *triIndices; /* triangle indices, cell by cell */
+    int totalIndices;
+    int totalCells;
+    float minX, minY, minZ;
+    float cellSizeX, cellSizeY, cellSizeZ;
+} CollisionGrid;
+
+/* simulation/lib/model_bounds.h:47-54.  All fields zero on allocation failure. */
+CollisionGrid buildCollisionGrid(GPUTriangle *tris, int numTris, float bminX, float bminY, float bminZ, float bmaxX,
+                                 float bmaxY, float bmaxZ);
 
 typedef struct {
     float scale;

@@ -844,6 +844,36 @@ void LBM_AdvectParticles(LBMGrid *g, Particle *p, int n, float dt, float velScal
         note_error(lbmdev_last_error());
 }
 
+void LBM_AdvectParticlesLive(LBMGrid *g, Particle *p, int n, float dt, const LBMParticleFlow *f, unsigned int seed,
+                             unsigned int step) {
+    if (!g || !p || !f || !live_dev(g)) return;
+    LbmParticleFlow d;
+    memset(&d, 0, sizeof(d));
+    const float wind[3] = {f->windX, f->windY, f->windZ};
+    const float lo[3] = {f->carMinX, f->carMinY, f->carMinZ}, hi[3] = {f->carMaxX, f->carMaxY, f->carMaxZ};
+    const float c[3] = {f->carCenterX, f->carCenterY, f->carCenterZ};
+    const float gm[3] = {f->gridMinX, f->gridMinY, f->gridMinZ};
+    const float gs[3] = {f->gridCellSizeX, f->gridCellSizeY, f->gridCellSizeZ};
+    memcpy(d.wind, wind, sizeof(wind));
+    memcpy(d.carMin, lo, sizeof(lo));
+    memcpy(d.carMax, hi, sizeof(hi));
+    memcpy(d.carCenter, c, sizeof(c));
+    memcpy(d.gridMin, gm, sizeof(gm));
+    memcpy(d.gridCellSize, gs, sizeof(gs));
+    d.collisionMode = f->collisionMode;
+    d.useLBM = f->useLBM;
+    d.vizMode = f->vizMode;
+    d.triangles = f->triangles;
+    d.numTriangles = f->numTriangles;
+    d.cellStart = (const int32_t *)f->cellStart;
+    d.cellCount = (const int32_t *)f->cellCount;
+    d.triIndices = (const int32_t *)f->triIndices;
+    d.numIndices = f->numIndices;
+    d.gridRes = f->gridRes;
+    if (f->collisionMode == 3 && !ensure_geometry(g)) return;  /* the solid buffer must be current */
+    if (!lbmdev_particles_live(impl_of(g)->dev, p, n, dt, &d, seed, step)) note_error(lbmdev_last_error());
+}
+
 void *LBM_DeviceAlloc(unsigned long long bytes) { return lbmdev_malloc((size_t)bytes); }
 void LBM_DeviceFree(void *p) { lbmdev_free(p); }
 int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes) {

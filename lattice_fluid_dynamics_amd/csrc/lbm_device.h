@@ -80,6 +80,20 @@ int lbmdev_export_populations(LbmDev *d, int periodicYZ, float *f_post_stream, f
 int lbmdev_total_mass(LbmDev *d, int periodicYZ, double *mass);
 
 int lbmdev_particles_seed(LbmDev *d, void *dev_particles, int n, uint32_t seed);
+/* Uniforms of the live particle update (simulation/shaders/particle.comp,
+ * set by main.c:1036-1096); the pointers are device memory. */
+typedef struct {
+    float wind[3], carMin[3], carMax[3], carCenter[3];
+    int collisionMode, useLBM, vizMode;
+    const float *triangles; /* 12 floats per triangle */
+    int numTriangles;
+    const int32_t *cellStart, *cellCount, *triIndices; /* the collision grid */
+    int numIndices;                                     /* entries of triIndices */
+    float gridMin[3], gridCellSize[3];
+    int gridRes;
+} LbmParticleFlow;
+int lbmdev_particles_live(LbmDev *d, void *dev_particles, int n, float dt, const LbmParticleFlow *flow,
+                          uint32_t seed, uint32_t step);
 int lbmdev_particles_advect(LbmDev *d, void *dev_particles, int n, float dt, float velScale,
                             uint32_t seed, uint32_t step);
 

@@ -1304,6 +1304,306 @@ __global__ __launch_bounds__(BLOCK) void k_particles_advect(Particle *p, int n, 
     p[i] = q;
 }
 
+// ---------------------------------------------------- live particle update
+// The particle update of the reference's running application
+// (simulation/shaders/particle.comp:218-407, dispatched by main.c:1012-1101):
+// nearest-cell velocity sampling, relaxation of the particle velocity towards
+// mix(wind, 8 u_lattice, 0.5), a forward bias outside the model's box, four
+// collision modes and the kill box.  The reference draws respawn positions
+// from a time-seeded sin hash; here the three uniforms come from the counter
+// hash of (seed, step, particle), so runs are reproducible.  Floating point
+// is evaluated in the order written, without contraction, as the oracle's
+// C restatement is.
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 v3add(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return V3{a.x + b.x, a.y + b.y, a.z + b.z};
+}
+__device__ __forceinline__ V3 v3sub(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return V3{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+__device__ __forceinline__ V3 v3scale(V3 a, float s) {
+#pragma clang fp contract(off)
+    return V3{a.x * s, a.y * s, a.z * s};
+}
+__device__ __forceinline__ float v3dot(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ float v3len(V3 a) { return sqrtf(v3dot(a, a)); }
+__device__ __forceinline__ V3 v3mix(V3 a, V3 b, float t) {
+#pragma clang fp contract(off)
+    return V3{a.x * (1.0f - t) + b.x * t, a.y * (1.0f - t) + b.y * t, a.z * (1.0f - t) + b.z * t};
+}
+__device__ __forceinline__ float gsign(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ float v3comp(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__device__ __forceinline__ void v3set(V3 &a, int k, float v) {
+    if (k == 0) a.x = v;
+    else if (k == 1) a.y = v;
+    else a.z = v;
+}
+
+// particle.comp:66-84 (and 86-103 for the density): the cell index of the
+// nearest-cell sampler, grid position clamped to [0.5, N - 1.5].
+__device__ __forceinline__ int live_cell_index(V3 p, int nx, int ny, int nz) {
+#pragma clang fp contract(off)
+    float gx = (p.x + 4.0f) / 8.0f * (float)nx;
+    float gy = (p.y + 2.0f) / 4.0f * (float)ny;
+    float gz = (p.z + 2.0f) / 4.0f * (float)nz;
+    gx = fminf(fmaxf(gx, 0.5f), (float)nx - 1.5f);
+    gy = fminf(fmaxf(gy, 0.5f), (float)ny - 1.5f);
+    gz = fminf(fmaxf(gz, 0.5f), (float)nz - 1.5f);
+    const int cx = min(max((int)floorf(gx), 0), nx - 1);
+    const int cy = min(max((int)floorf(gy), 0), ny - 1);
+    const int cz = min(max((int)floorf(gz), 0), nz - 1);
+    const long long idx = cx + (long long)cy * nx + (long long)cz * nx * ny;
+    const long long maxIdx = (long long)nx * ny * nz - 1;
+    return (int)(idx < 0 ? 0 : (idx > maxIdx ? maxIdx : idx));
+}
+
+__device__ __forceinline__ bool inside_box(V3 p, V3 lo, V3 hi) {
+    return p.x >= lo.x && p.x <= hi.x && p.y >= lo.y && p.y <= hi.y && p.z >= lo.z && p.z <= hi.z;
+}
+
+// particle.comp:112-125: the box face normal nearest to the particle.
+__device__ __forceinline__ V3 box_normal(V3 p, V3 lo, V3 hi, V3 c) {
+#pragma clang fp contract(off)
+    const V3 t = v3sub(p, c);
+    const V3 a = v3(fabsf(t.x), fabsf(t.y), fabsf(t.z));
+    const V3 h = v3scale(v3sub(hi, lo), 0.5f);
+    V3 n = v3(0.0f, 0.0f, 0.0f);
+    if (a.x / h.x > a.y / h.y && a.x / h.x > a.z / h.z) n.x = gsign(t.x);
+    else if (a.y / h.y > a.z / h.z) n.y = gsign(t.y);
+    else n.z = gsign(t.z);
+    return n;
+}
+
+// particle.comp:127-145: out through the face of least penetration, 0.03 beyond.
+__device__ __forceinline__ V3 box_push_out(V3 p, V3 lo, V3 hi) {
+#pragma clang fp contract(off)
+    const V3 c = v3scale(v3add(lo, hi), 0.5f);
+    const V3 h = v3scale(v3sub(hi, lo), 0.5f);
+    const V3 t = v3sub(p, c);
+    const V3 pen = v3(h.x - fabsf(t.x), h.y - fabsf(t.y), h.z - fabsf(t.z));
+    if (pen.x < pen.y && pen.x < pen.z) p.x = c.x + gsign(t.x) * (h.x + 0.03f);
+    else if (pen.y < pen.z) p.y = c.y + gsign(t.y) * (h.y + 0.03f);
+    else p.z = c.z + gsign(t.z) * (h.z + 0.03f);
+    return p;
+}
+
+// Distance from p to triangle (v0, v1, v2), its closest point and unit normal:
+// the region classification of the minimiser of |v0 + s e0 + t e1 - p|^2
+// over the triangle (D. Eberly, "Distance Between Point and Triangle in 3D",
+// Geometric Tools), as particle.comp:147-216 evaluates it.
+__device__ __forceinline__ float point_triangle(V3 p, V3 v0, V3 v1, V3 v2, V3 &closest, V3 &normal) {
+#pragma clang fp contract(off)
+    const V3 e0 = v3sub(v1, v0), e1 = v3sub(v2, v0), d0 = v3sub(v0, p);
+    const float a = v3dot(e0, e0), b = v3dot(e0, e1), c = v3dot(e1, e1);
+    const float d = v3dot(e0, d0), e = v3dot(e1, d0);
+    const float det = a * c - b * b;
+    float s = b * e - c * d, t = b * d - a * e;
+    if (s + t <= det) {
+        if (s < 0.0f) {
+            if (t < 0.0f) {
+                if (d < 0.0f) {
+                    t = 0.0f;
+                    s = fminf(fmaxf(-d / a, 0.0f), 1.0f);
+                } else {
+                    s = 0.0f;
+                    t = fminf(fmaxf(-e / c, 0.0f), 1.0f);
+                }
+            } else {
+                s = 0.0f;
+                t = fminf(fmaxf(-e / c, 0.0f), 1.0f);
+            }
+        } else if (t < 0.0f) {
+            t = 0.0f;
+            s = fminf(fmaxf(-d / a, 0.0f), 1.0f);
+        } else {
+            const float inv = 1.0f / det;
+            s *= inv;
+            t *= inv;
+        }
+    } else {
+        if (s < 0.0f) {
+            const float t0 = b + d, t1 = c + e;
+            if (t1 > t0) {
+                s = fminf(fmaxf((t1 - t0) / (a - 2.0f * b + c), 0.0f), 1.0f);
+                t = 1.0f - s;
+            } else {
+                s = 0.0f;
+                t = fminf(fmaxf(-e / c, 0.0f), 1.0f);
+            }
+        } else if (t < 0.0f) {
+            const float t0 = b + e, t1 = a + d;
+            if (t1 > t0) {
+                t = fminf(fmaxf((t1 - t0) / (a - 2.0f * b + c), 0.0f), 1.0f);
+                s = 1.0f - t;
+            } else {
+                t = 0.0f;
+                s = fminf(fmaxf(-d / a, 0.0f), 1.0f);
+            }
+        } else {
+            s = fminf(fmaxf(((c + e) - (b + d)) / (a - 2.0f * b + c), 0.0f), 1.0f);
+            t = 1.0f - s;
+        }
+    }
+    closest = v3add(v3add(v0, v3scale(e0, s)), v3scale(e1, t));
+    const V3 cr = v3(e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x);
+    const float cl = v3len(cr);
+    normal = v3(cr.x / cl, cr.y / cl, cr.z / cl);
+    return v3len(v3sub(p, closest));
+}
+
+struct LiveArgs {
+    V3 wind, carMin, carMax, carCenter;
+    int mode, useLBM, vizMode;
+    const float *tris;
+    int ntri;
+    const int32_t *cellStart, *cellCount, *triIdx;
+    int nIdx;  // entries of triIdx
+    V3 gridMin, gridCell;
+    int gridRes;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_particles_live(Particle *pp, int n, const float4 *vel,
+                                                          const int32_t *solid, int nx, int ny, int nz, float dt,
+                                                          LiveArgs f, uint32_t seed, uint32_t step) {
+#pragma clang fp contract(off)
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= n) return;
+    Particle q = pp[idx];
+    V3 pos = v3(q.x, q.y, q.z), v = v3(q.vx, q.vy, q.vz);
+    float pad = q.pad, life = q.life;
+    V3 flow = f.wind;
+    if (f.useLBM) {
+        const float4 s = vel[live_cell_index(pos, nx, ny, nz)];
+        flow = v3mix(f.wind, v3(s.x * 8.0f, s.y * 8.0f, s.z * 8.0f), 0.5f);
+    }
+    if (f.vizMode == 8 && f.useLBM) {
+        const float rho = vel[live_cell_index(pos, nx, ny, nz)].w;
+        const float cp = (rho - 1.0f) / (0.5f * 0.05f * 0.05f / (1.0f / 3.0f));
+        pad = pad * (1.0f - 0.05f) + cp * 0.05f;
+    } else {
+        pad = pad * (1.0f - 0.02f) + v3len(flow) * 0.02f;
+    }
+    v = v3mix(v, flow, 0.05f);
+    const V3 pad5 = v3(0.05f, 0.05f, 0.05f);
+    const bool near = f.mode > 0 && inside_box(pos, v3sub(f.carMin, pad5), v3add(f.carMax, pad5));
+    if (!near) v.x = fmaxf(v.x, f.wind.x * 0.3f);
+    pos = v3add(pos, v3scale(v, dt));
+
+    if (f.mode == 1) {
+        if (inside_box(pos, f.carMin, f.carMax)) {
+            const V3 nrm = box_normal(pos, f.carMin, f.carMax, f.carCenter);
+            pos = box_push_out(pos, f.carMin, f.carMax);
+            const float vdn = v3dot(v, nrm);
+            if (vdn < 0.0f) {
+                v = v3sub(v, v3scale(nrm, 1.05f * vdn));
+                V3 away = v3sub(pos, f.carCenter);
+                away.x = 0.0f;
+                const float al = v3len(away);
+                if (al > 0.001f) v = v3add(v, v3scale(v3(away.x / al, away.y / al, away.z / al), fabsf(vdn) * 0.4f));
+            }
+        }
+    } else if (f.mode == 2 && f.ntri > 0 && f.gridRes > 0) {
+        const float radius = 0.03f;
+        const V3 rel = v3(v3sub(pos, f.gridMin).x / f.gridCell.x, v3sub(pos, f.gridMin).y / f.gridCell.y,
+                          v3sub(pos, f.gridMin).z / f.gridCell.z);
+        const int cx = (int)floorf(rel.x), cy = (int)floorf(rel.y), cz = (int)floorf(rel.z);
+        const int R = f.gridRes;
+        if (cx >= -1 && cx <= R && cy >= -1 && cy <= R && cz >= -1 && cz <= R) {
+            const int lx = max(cx - 1, 0), ly = max(cy - 1, 0), lz = max(cz - 1, 0);
+            const int hx = min(cx + 1, R - 1), hy = min(cy + 1, R - 1), hz = min(cz + 1, R - 1);
+            bool hit = false;
+            for (int z = lz; z <= hz && !hit; z++)
+                for (int y = ly; y <= hy && !hit; y++)
+                    for (int x = lx; x <= hx && !hit; x++) {
+                        const int ci = x + y * R + z * R * R;
+                        const int start = f.cellStart[ci], cnt = f.cellCount[ci];
+                        for (int j = 0; j < cnt && !hit; j++) {
+                            // entries outside the arrays (an inconsistent grid) are skipped
+                            if (start + j < 0 || start + j >= f.nIdx) continue;
+                            const int ti = f.triIdx[start + j];
+                            if (ti < 0 || ti >= f.ntri) continue;
+                            const float *t = f.tris + 12 * (long long)ti;
+                            V3 cp, nrm;
+                            const float dist = point_triangle(pos, v3(t[0], t[1], t[2]), v3(t[4], t[5], t[6]),
+                                                              v3(t[8], t[9], t[10]), cp, nrm);
+                            if (dist < radius) {
+                                pos = v3add(cp, v3scale(nrm, radius + 0.001f));
+                                const float vdn = v3dot(v, nrm);
+                                if (vdn < 0.0f) v = v3sub(v, v3scale(nrm, 1.05f * vdn));
+                                hit = true;
+                            }
+                        }
+                    }
+        }
+    } else if (f.mode == 3) {
+        const float gx = (pos.x + 4.0f) / 8.0f * (float)nx, gy = (pos.y + 2.0f) / 4.0f * (float)ny,
+                    gz = (pos.z + 2.0f) / 4.0f * (float)nz;
+        const int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
+        if (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz &&
+            solid[ix + (long long)iy * nx + (long long)iz * nx * ny] == 1) {
+            const V3 cell = v3(8.0f / (float)nx, 4.0f / (float)ny, 4.0f / (float)nz);
+            const V3 ctr = v3(((float)ix + 0.5f) / (float)nx * 8.0f - 4.0f, ((float)iy + 0.5f) / (float)ny * 4.0f - 2.0f,
+                              ((float)iz + 0.5f) / (float)nz * 4.0f - 2.0f);
+            const V3 toP = v3sub(pos, ctr);
+            const V3 ab = v3(fabsf(toP.x) / (cell.x * 0.5f), fabsf(toP.y) / (cell.y * 0.5f),
+                             fabsf(toP.z) / (cell.z * 0.5f));
+            // axes by penetration depth: the largest |offset| (the shallowest) first
+            int ax[3] = {0, 1, 2};
+            if (v3comp(ab, ax[0]) < v3comp(ab, ax[1])) { const int t = ax[0]; ax[0] = ax[1]; ax[1] = t; }
+            if (v3comp(ab, ax[1]) < v3comp(ab, ax[2])) { const int t = ax[1]; ax[1] = ax[2]; ax[2] = t; }
+            if (v3comp(ab, ax[0]) < v3comp(ab, ax[1])) { const int t = ax[0]; ax[0] = ax[1]; ax[1] = t; }
+            bool pushed = false;
+            const int dims[3] = {nx, ny, nz};
+            for (int k = 0; k < 3 && !pushed; k++) {
+                const int a = ax[k];
+                V3 nrm = v3(0.0f, 0.0f, 0.0f);
+                v3set(nrm, a, gsign(v3comp(toP, a)));
+                const int ni[3] = {ix + (int)nrm.x, iy + (int)nrm.y, iz + (int)nrm.z};
+                if (ni[a] >= 0 && ni[a] < dims[a] &&
+                    solid[ni[0] + (long long)ni[1] * nx + (long long)ni[2] * nx * ny] == 1)
+                    continue;  // the neighbour on that side is solid too: try the next axis
+                pos = v3add(ctr, v3(nrm.x * cell.x * 0.52f, nrm.y * cell.y * 0.52f, nrm.z * cell.z * 0.52f));
+                const float vdn = v3dot(v, nrm);
+                if (vdn < 0.0f) v = v3sub(v, v3scale(nrm, 1.05f * vdn));
+                pushed = true;
+            }
+            if (!pushed) life = -1.0f;
+        }
+    }
+    // kill box (particle.comp:369-405)
+    const float killX = f.carMax.x + 2.0f, tunnel = 1.3f;
+    pos.y = fminf(fmaxf(pos.y, -tunnel), tunnel);
+    pos.z = fminf(fmaxf(pos.z, -tunnel), tunnel);
+    const float speed = v3len(v);
+    const bool stuck = speed < 0.05f && pos.x > f.carMin.x - 0.5f && pos.x < f.carMax.x + 0.5f;
+    if (pos.x > killX || stuck || life < 0.0f) {
+        const uint32_t base = mix32(step ^ mix32(seed + 0x5bd1e995u));
+        const float ry = u01(mix32(3u * (uint32_t)idx + base)) - 0.5f;
+        const float rz = u01(mix32(3u * (uint32_t)idx + 1u + base)) - 0.5f;
+        const float rv = u01(mix32(3u * (uint32_t)idx + 2u + base));
+        pos = v3(-4.0f, ry * 2.0f * tunnel, rz * 2.0f * tunnel);
+        v = v3(0.2f + rv * 0.15f, 0.0f, 0.0f);
+        life = 1.0f;
+    }
+    q.x = pos.x;
+    q.y = pos.y;
+    q.z = pos.z;
+    q.pad = pad;
+    q.vx = v.x;
+    q.vy = v.y;
+    q.vz = v.z;
+    q.life = life;
+    pp[idx] = q;
+}
+
 }  // namespace
 
 // ======================================================================
@@ -2046,6 +2346,37 @@ extern "C" int lbmdev_particles_seed(LbmDev *d, void *dev_particles, int n, uint
     hipLaunchKernelGGL(k_particles_seed, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, d->stream,
                        (Particle *)dev_particles, n, seed);
     HIPCHECK(hipGetLastError(), "particle seed launch");
+    return 1;
+}
+
+extern "C" int lbmdev_particles_live(LbmDev *d, void *dev_particles, int n, float dt, const LbmParticleFlow *fl,
+                                    uint32_t seed, uint32_t step) {
+    if (n <= 0) return 1;
+    LiveArgs a;
+    a.wind = V3{fl->wind[0], fl->wind[1], fl->wind[2]};
+    a.carMin = V3{fl->carMin[0], fl->carMin[1], fl->carMin[2]};
+    a.carMax = V3{fl->carMax[0], fl->carMax[1], fl->carMax[2]};
+    a.carCenter = V3{fl->carCenter[0], fl->carCenter[1], fl->carCenter[2]};
+    a.mode = fl->collisionMode;
+    a.useLBM = fl->useLBM;
+    a.vizMode = fl->vizMode;
+    a.tris = fl->triangles;
+    a.ntri = fl->numTriangles;
+    a.cellStart = fl->cellStart;
+    a.cellCount = fl->cellCount;
+    a.triIdx = fl->triIndices;
+    a.nIdx = fl->numIndices;
+    a.gridMin = V3{fl->gridMin[0], fl->gridMin[1], fl->gridMin[2]};
+    a.gridCell = V3{fl->gridCellSize[0], fl->gridCellSize[1], fl->gridCellSize[2]};
+    a.gridRes = fl->gridRes;
+    if (a.mode == 2 && a.ntri > 0 && a.gridRes > 0 && (!a.tris || !a.cellStart || !a.cellCount || !a.triIdx)) {
+        snprintf(g_err, sizeof(g_err), "collision mode 2 needs the triangles and the collision grid on the device");
+        return 0;
+    }
+    hipLaunchKernelGGL(k_particles_live, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, d->stream,
+                       (Particle *)dev_particles, n, (const float4 *)d->vel, (const int32_t *)d->solid, d->nx, d->ny,
+                       d->nz, dt, a, seed, step);
+    HIPCHECK(hipGetLastError(), "live particle launch");
     return 1;
 }
 

@@ -190,6 +190,78 @@ GPUTriangle *createTriangleBuffer(Model *model, float scale, float offsetX, floa
     return t;
 }
 
+/* The cell range of triangle t along one axis: floor of its extent relative
+ * to the grid origin over the cell size, clamped to [0, R - 1] (the
+ * expressions of model_bounds.c:45-73). */
+static void tri_range(const float *t, int axis, float origin, float cell, int R, int *lo, int *hi) {
+    const float a = t[axis], b = t[4 + axis], c = t[8 + axis];
+    const float mn = fminf(a, fminf(b, c)), mx = fmaxf(a, fmaxf(b, c));
+    *lo = (int)floorf((mn - origin) / cell);
+    *hi = (int)floorf((mx - origin) / cell);
+    if (*lo < 0) *lo = 0;
+    if (*hi >= R) *hi = R - 1;
+}
+
+CollisionGrid buildCollisionGrid(GPUTriangle *tris, int numTris, float bminX, float bminY, float bminZ,
+                                 float bmaxX, float bmaxY, float bmaxZ) {
+    CollisionGrid g;
+    memset(&g, 0, sizeof(g));
+    const int R = COLL_GRID_RES;
+    const float pad = 0.05f;
+    g.totalCells = R * R * R;
+    g.minX = bminX - pad;
+    g.minY = bminY - pad;
+    g.minZ = bminZ - pad;
+    g.cellSizeX = ((bmaxX + pad) - g.minX) / R;
+    g.cellSizeY = ((bmaxY + pad) - g.minY) / R;
+    g.cellSizeZ = ((bmaxZ + pad) - g.minZ) / R;
+    const float origin[3] = {g.minX, g.minY, g.minZ}, cell[3] = {g.cellSizeX, g.cellSizeY, g.cellSizeZ};
+    /* each triangle's clamped cell box, once */
+    int *box = (int *)malloc(sizeof(int) * 6 * (size_t)(numTris > 0 ? numTris : 1));
+    g.cellCount = (int *)calloc((size_t)g.totalCells, sizeof(int));
+    g.cellStart = (int *)malloc(sizeof(int) * (size_t)g.totalCells);
+    int *fill = (int *)calloc((size_t)g.totalCells, sizeof(int));
+    if (!box || !g.cellCount || !g.cellStart || !fill) goto fail;
+    long long total = 0;
+    for (int t = 0; t < numTris; t++) {
+        int *b = box + 6 * t;
+        for (int a = 0; a < 3; a++) tri_range((const float *)&tris[t], a, origin[a], cell[a], R, &b[2 * a], &b[2 * a + 1]);
+        for (int z = b[4]; z <= b[5]; z++)
+            for (int y = b[2]; y <= b[3]; y++)
+                for (int x = b[0]; x <= b[1]; x++) {
+                    g.cellCount[x + y * R + z * R * R]++;
+                    total++;
+                }
+    }
+    if (total > 0x7fffffff) goto fail;
+    g.totalIndices = (int)total;
+    g.cellStart[0] = 0;
+    for (int c = 1; c < g.totalCells; c++) g.cellStart[c] = g.cellStart[c - 1] + g.cellCount[c - 1];
+    /* triangle order within each cell, as the reference's second pass */
+    g.triIndices = (int *)malloc(sizeof(int) * (size_t)(total > 0 ? total : 1));
+    if (!g.triIndices) goto fail;
+    for (int t = 0; t < numTris; t++) {
+        const int *b = box + 6 * t;
+        for (int z = b[4]; z <= b[5]; z++)
+            for (int y = b[2]; y <= b[3]; y++)
+                for (int x = b[0]; x <= b[1]; x++) {
+                    const int c = x + y * R + z * R * R;
+                    g.triIndices[g.cellStart[c] + fill[c]++] = t;
+                }
+    }
+    free(box);
+    free(fill);
+    return g;
+fail:
+    free(box);
+    free(fill);
+    free(g.cellCount);
+    free(g.cellStart);
+    free(g.triIndices);
+    memset(&g, 0, sizeof(g));
+    return g;
+}
+
 ModelPlacement LBM_AutoPlaceModel(const Model *model, float scale, int scaleGiven) {
     ModelPlacement mp;
     mp.scale = scaleGiven ? scale : 1.0f;

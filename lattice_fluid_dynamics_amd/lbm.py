@@ -87,6 +87,7 @@ SIGNATURES = {
     "LBM_Synchronize": (None, [GridP]),
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
+    "LBM_AdvectParticlesLive": (None, [GridP, c_void_p, c_int, c_float, c_void_p, ctypes.c_uint, ctypes.c_uint]),
     "LBM_GetGuardResetCount": (ctypes.c_longlong, [GridP]),
     "LBM_GetLastError": (ctypes.c_char_p, []),
     "LBM_CdMonitorCreate": (c_void_p, [GridP, c_float, c_float, c_int]),
@@ -356,6 +357,65 @@ def LBM_GetGuardResetCount(g: Grid) -> int:
     if n < 0:
         raise RuntimeError("LBM_GetGuardResetCount failed: " + last_error())
     return n
+
+
+class LBMParticleFlow(ctypes.Structure):
+    """include/lbm.h LBMParticleFlow: the uniforms of the live particle update."""
+    _fields_ = ([(n, c_float) for n in ("windX", "windY", "windZ", "carMinX", "carMinY", "carMinZ", "carMaxX",
+                                        "carMaxY", "carMaxZ", "carCenterX", "carCenterY", "carCenterZ")]
+                + [("collisionMode", c_int), ("useLBM", c_int), ("vizMode", c_int),
+                   ("triangles", c_void_p), ("numTriangles", c_int), ("cellStart", c_void_p),
+                   ("cellCount", c_void_p), ("triIndices", c_void_p), ("numIndices", c_int)]
+                + [(n, c_float) for n in ("gridMinX", "gridMinY", "gridMinZ", "gridCellSizeX", "gridCellSizeY",
+                                          "gridCellSizeZ")]
+                + [("gridRes", c_int)])
+
+
+def particle_flow(wind, car_min, car_max, car_center, collision_mode=0, use_lbm=1, viz_mode=0, triangles=None,
+                  grid=None, grid_res=8):
+    """An LBMParticleFlow; for collision mode 2, `triangles` and the arrays of
+    `grid` (model.collision_grid) are device buffers: (ptr, count) pairs as
+    DeviceBuffer gives them, and grid["origin"], grid["cell"] the geometry."""
+    f = LBMParticleFlow()
+    f.windX, f.windY, f.windZ = wind
+    f.carMinX, f.carMinY, f.carMinZ = car_min
+    f.carMaxX, f.carMaxY, f.carMaxZ = car_max
+    f.carCenterX, f.carCenterY, f.carCenterZ = car_center
+    f.collisionMode, f.useLBM, f.vizMode = collision_mode, use_lbm, viz_mode
+    if triangles is not None:
+        f.triangles, f.numTriangles = triangles
+    if grid is not None:
+        f.cellStart = grid["cellStart_dev"]
+        f.cellCount = grid["cellCount_dev"]
+        f.triIndices, f.numIndices = grid["triIndices_dev"]
+        f.gridMinX, f.gridMinY, f.gridMinZ = [float(v) for v in grid["origin"]]
+        f.gridCellSizeX, f.gridCellSizeY, f.gridCellSizeZ = [float(v) for v in grid["cell"]]
+        f.gridRes = grid_res
+    return f
+
+
+class DeviceBuffer:
+    """Device copy of a host array (LBM_DeviceAlloc / LBM_CopyToDevice)."""
+
+    def __init__(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self.n = a.shape[0] if a.ndim else 1
+        self.ptr = load_library().LBM_DeviceAlloc(max(a.nbytes, 4))
+        if not self.ptr or (a.nbytes and not load_library().LBM_CopyToDevice(self.ptr, a.ctypes.data, a.nbytes)):
+            raise MemoryError("device buffer")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                load_library().LBM_DeviceFree(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def LBM_AdvectParticlesLive(g: Grid, parts: "DeviceParticles", dt: float, flow: LBMParticleFlow, seed: int,
+                            step: int) -> None:
+    load_library().LBM_AdvectParticlesLive(g.ptr, parts.ptr, parts.n, dt, ctypes.byref(flow), seed, step)
 
 
 class DeviceParticles:

@@ -42,6 +42,13 @@ class CarBounds(ctypes.Structure):
                 ("minX", "minY", "minZ", "maxX", "maxY", "maxZ", "centerX", "centerY", "centerZ")]
 
 
+class CollisionGrid(ctypes.Structure):
+    _fields_ = [("cellStart", ctypes.c_void_p), ("cellCount", ctypes.c_void_p), ("triIndices", ctypes.c_void_p),
+                ("totalIndices", ctypes.c_int), ("totalCells", ctypes.c_int),
+                ("minX", ctypes.c_float), ("minY", ctypes.c_float), ("minZ", ctypes.c_float),
+                ("cellSizeX", ctypes.c_float), ("cellSizeY", ctypes.c_float), ("cellSizeZ", ctypes.c_float)]
+
+
 class ModelPlacement(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("scale", "offsetX", "offsetY", "offsetZ", "rotationY")]
 
@@ -62,6 +69,7 @@ SIGNATURES = {
     "Voxelize_WriteBinary": (ctypes.c_int, [ctypes.POINTER(VoxelGrid), ctypes.c_char_p]),
     "Voxelize_SolidFraction": (_F, [ctypes.POINTER(VoxelGrid)]),
     "Voxelize_Free": (None, [ctypes.POINTER(VoxelGrid)]),
+    "buildCollisionGrid": (CollisionGrid, [ctypes.c_void_p, ctypes.c_int, _F, _F, _F, _F, _F, _F]),
 }
 PLACEMENT_SIGNATURE = (ModelPlacement, [ctypes.POINTER(Model), _F, ctypes.c_int])
 
@@ -149,6 +157,35 @@ class LoadedModel:
             self.close()
         except Exception:
             pass
+
+
+COLL_GRID_RES = 8
+
+
+def collision_grid(triangles: np.ndarray, bbox, lib: Optional[ctypes.CDLL] = None):
+    """buildCollisionGrid over (n, 12) placed triangles and the model box:
+    a dict of cellStart, cellCount, triIndices (int32 arrays, R^3 cells of
+    R = 8), origin and cell size (float32 triples)."""
+    L = _lib(lib)
+    t = np.ascontiguousarray(triangles, np.float32)
+    g = L.buildCollisionGrid(t.ctypes.data, t.shape[0], *[float(v) for v in bbox])
+    if g.totalCells and not (g.cellStart and g.cellCount):
+        raise MemoryError("buildCollisionGrid failed")
+
+    def take(ptr, n):
+        if not ptr or n == 0:
+            return np.zeros(0, np.int32)
+        out = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int)), (n,)).copy()
+        _libc.free(ptr)
+        return out
+
+    out = {"cellStart": take(g.cellStart, g.totalCells), "cellCount": take(g.cellCount, g.totalCells)}
+    out["triIndices"] = take(g.triIndices, g.totalIndices) if g.triIndices else np.zeros(0, np.int32)
+    if g.triIndices and g.totalIndices == 0:
+        _libc.free(g.triIndices)
+    out["origin"] = np.array([g.minX, g.minY, g.minZ], np.float32)
+    out["cell"] = np.array([g.cellSizeX, g.cellSizeY, g.cellSizeZ], np.float32)
+    return out
 
 
 def auto_placement(model: LoadedModel, scale: float = 0.0) -> Tuple[float, float, float, float, float]:

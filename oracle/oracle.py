@@ -283,6 +283,55 @@ def particles_seed(n: int, seed: int) -> np.ndarray:
     return p
 
 
+class OrParticleFlow(ctypes.Structure):
+    """oracle_particles.c OrParticleFlow (host pointers)."""
+    _fields_ = [("wind", ctypes.c_float * 3), ("carMin", ctypes.c_float * 3), ("carMax", ctypes.c_float * 3),
+                ("carCenter", ctypes.c_float * 3), ("collisionMode", ctypes.c_int), ("useLBM", ctypes.c_int),
+                ("vizMode", ctypes.c_int), ("triangles", ctypes.c_void_p), ("numTriangles", ctypes.c_int),
+                ("cellStart", ctypes.c_void_p), ("cellCount", ctypes.c_void_p), ("triIndices", ctypes.c_void_p),
+                ("numIndices", ctypes.c_int), ("gridMin", ctypes.c_float * 3), ("gridCellSize", ctypes.c_float * 3),
+                ("gridRes", ctypes.c_int)]
+
+
+def particles_live(vel: np.ndarray, solid: np.ndarray, nx, ny, nz, parts: np.ndarray, dt: float, wind, car_min,
+                   car_max, car_center, collision_mode=0, use_lbm=1, viz_mode=0, triangles=None, grid=None,
+                   grid_res=8, seed=0, step=0) -> np.ndarray:
+    """One live update (simulation/shaders/particle.comp:218-407) of (n, 8) particles."""
+    v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+    s = np.ascontiguousarray(solid, dtype=np.int32).reshape(-1)
+    p = np.ascontiguousarray(parts, dtype=np.float32).copy()
+    f = OrParticleFlow()
+    f.wind[:], f.carMin[:], f.carMax[:], f.carCenter[:] = wind, car_min, car_max, car_center
+    f.collisionMode, f.useLBM, f.vizMode = collision_mode, use_lbm, viz_mode
+    keep = []
+    if triangles is not None:
+        t = np.ascontiguousarray(triangles, np.float32)
+        keep.append(t)
+        f.triangles, f.numTriangles = t.ctypes.data, t.shape[0]
+    if grid is not None:
+        arrs = [np.ascontiguousarray(grid[k], np.int32) for k in ("cellStart", "cellCount", "triIndices")]
+        keep += arrs
+        f.cellStart, f.cellCount, f.triIndices = [a.ctypes.data for a in arrs]
+        f.numIndices = arrs[2].size
+        f.gridMin[:], f.gridCellSize[:] = [float(x) for x in grid["origin"]], [float(x) for x in grid["cell"]]
+        f.gridRes = grid_res
+    lib().or_particles_live(_ptr(v, ctypes.c_float), _ptr(s, ctypes.c_int), nx, ny, nz, _ptr(p, ctypes.c_float),
+                            p.shape[0], ctypes.c_float(dt), ctypes.byref(f), ctypes.c_uint32(seed),
+                            ctypes.c_uint32(step))
+    return p
+
+
+def point_triangle(p, v0, v1, v2):
+    """(distance, closest point, unit normal) of oracle_particles.c."""
+    a = [np.ascontiguousarray(x, np.float32) for x in (p, v0, v1, v2)]
+    c = np.zeros(3, np.float32)
+    n = np.zeros(3, np.float32)
+    lib().or_point_triangle.restype = ctypes.c_float
+    d = lib().or_point_triangle(*[_ptr(x, ctypes.c_float) for x in a], _ptr(c, ctypes.c_float),
+                                _ptr(n, ctypes.c_float))
+    return float(d), c, n
+
+
 def particles_advect(vel: np.ndarray, nx, ny, nz, parts: np.ndarray, dt: float, vel_scale: float, seed: int,
                      step: int) -> np.ndarray:
     v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)

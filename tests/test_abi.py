@@ -14,7 +14,8 @@ from lattice_fluid_dynamics_amd import lbm, model, superres, warmstart
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm.h", "lbm_warmstart.h", "lbm_model.h", "lbm_superres.h")]
 # lbm_model.h and lbm_superres.h keep the reference helpers' own names
-MODEL_NAMES = r"loadOBJ|freeModel|computeModelBounds|createTriangleBuffer|Voxelize_[A-Za-z]+|SR_[A-Za-z]+"
+MODEL_NAMES = (r"loadOBJ|freeModel|computeModelBounds|createTriangleBuffer|buildCollisionGrid|Voxelize_[A-Za-z]+|"
+               r"SR_[A-Za-z]+")
 
 
 def declared_functions(headers=HEADERS):
@@ -80,7 +81,7 @@ def test_particle_record_is_32_bytes():
 def test_model_records_match_c():
     """Model, CarBounds, GPUTriangle (48 bytes), ModelPlacement and VoxelGrid:
     the ctypes mirrors have the C sizes (simulation/lib/model_bounds.h:8-18)."""
-    names = ["Model", "CarBounds", "GPUTriangle", "ModelPlacement", "VoxelGrid", "Vertex", "Face"]
+    names = ["Model", "CarBounds", "GPUTriangle", "ModelPlacement", "VoxelGrid", "Vertex", "Face", "CollisionGrid"]
     src = '#include <stdio.h>\n#include "lbm_model.h"\nint main(void){\n'
     src += "".join(f'printf("%zu\\n", sizeof({n}));\n' for n in names) + "return 0;}\n"
     with tempfile.TemporaryDirectory() as d:
@@ -91,6 +92,24 @@ def test_model_records_match_c():
         out = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     py = [ctypes.sizeof(getattr(model, n)) if hasattr(model, n) else 48 for n in names]
     assert out == py and out[2] == 48
+
+
+def test_particle_flow_layout_matches_c():
+    """LBMParticleFlow (include/lbm.h): size and every field offset of the
+    ctypes mirror as the C compiler lays them out."""
+    fields = [f for f, _ in lbm.LBMParticleFlow._fields_]
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "lbm.h"\nint main(void){\n'
+    src += 'printf("%zu\\n", sizeof(LBMParticleFlow));\n'
+    src += "".join(f'printf("%zu\\n", offsetof(LBMParticleFlow, {f}));\n' for f in fields) + "return 0;}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(lbm.LBMParticleFlow)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(lbm.LBMParticleFlow, f).offset == off, f
 
 
 def test_create_without_device_fails_loudly(monkeypatch):

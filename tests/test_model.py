@@ -187,3 +187,57 @@ def test_voxelizer_closed_forms(tmp_path):
     raw = open(path, "rb").read()
     assert raw[:4] == b"VOXB" and np.frombuffer(raw[4:12], "<u4").tolist() == [1, 20]
     assert np.array_equal(np.frombuffer(raw[12:], np.uint8).reshape(20, 20, 20), g)
+
+
+def grid_equal(a, b):
+    return (all(np.array_equal(a[k], b[k]) for k in ("cellStart", "cellCount", "triIndices"))
+            and np.array_equal(bits(a["origin"]), bits(b["origin"])) and np.array_equal(bits(a["cell"]), bits(b["cell"])))
+
+
+@have_ref
+@pytest.mark.parametrize("key", list(MESHES))
+def test_collision_grid_matches_reference_build(ref, key):
+    """buildCollisionGrid (simulation/src/model_bounds.c:10-137) over the
+    reference's own assets as its caller places them, and over the car proxy:
+    cell offsets, counts, triangle lists, origin and cell size bit-exact."""
+    from lattice_fluid_dynamics_amd import geometry
+    tris, bbox = REAL_TRIS(key)
+    a, b = M.collision_grid(tris, bbox), M.collision_grid(tris, bbox, lib=ref)
+    assert a["triIndices"].size > tris.shape[0] and grid_equal(a, b)
+    tris, bbox = geometry.car_proxy(seed=0)
+    assert grid_equal(M.collision_grid(tris, bbox), M.collision_grid(tris, bbox, lib=ref))
+
+
+def REAL_TRIS(key):
+    return GOLDEN[key + "_tris"], tuple(float(v) for v in GOLDEN[key + "_bbox"])
+
+
+def test_collision_grid_closed_form():
+    """The 12 triangles of an axis box over its own bounds: the grid spans
+    the box grown by 0.05 in 8 cells per axis; each triangle is listed, in
+    triangle order, in exactly the cells its bounding box covers (clamped),
+    cellStart is the prefix sum of cellCount and every cell holds only
+    triangles that touch it."""
+    from tests.test_oracle import box_mesh
+    lo, hi = (-1.0, -0.5, -0.25), (1.0, 0.5, 0.75)
+    tris = box_mesh(lo, hi)
+    g = M.collision_grid(tris, (*lo, *hi))
+    R = M.COLL_GRID_RES
+    assert np.allclose(g["origin"], np.array(lo) - 0.05)
+    assert np.allclose(g["cell"], (np.array(hi) - np.array(lo) + 0.1) / R)
+    assert np.array_equal(g["cellStart"][1:], np.cumsum(g["cellCount"])[:-1]) and g["cellStart"][0] == 0
+    expect = [[] for _ in range(R ** 3)]
+    for t in range(12):
+        v = tris[t].reshape(3, 4)[:, :3]
+        r = [np.clip(np.floor((v[:, a].min() - g["origin"][a]) / g["cell"][a]), 0, R - 1).astype(int) for a in range(3)]
+        s = [np.clip(np.floor((v[:, a].max() - g["origin"][a]) / g["cell"][a]), 0, R - 1).astype(int) for a in range(3)]
+        for z in range(r[2], s[2] + 1):
+            for y in range(r[1], s[1] + 1):
+                for x in range(r[0], s[0] + 1):
+                    expect[x + y * R + z * R * R].append(t)
+    for c in range(R ** 3):
+        got = g["triIndices"][g["cellStart"][c]:g["cellStart"][c] + g["cellCount"][c]].tolist()
+        assert got == expect[c]
+    # a face triangle lies in one layer of cells across its normal: the -z
+    # face (z = lo) in layer 0, the +z face (z = hi) in layer R - 1
+    assert all(0 in expect[x + y * R] or 1 in expect[x + y * R] for x in range(1, R - 1) for y in range(1, R - 1))
