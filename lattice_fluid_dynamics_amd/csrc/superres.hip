@@ -31,13 +31,13 @@
 #include <string.h>
 
 #include "../../include/lbm_superres.h"
+#include "ltws.h"
 
 namespace {
 
 constexpr int H = 128, IN = 108, INP = 112, OUT = 32;
 constexpr int LD = 132;  // activation row stride (floats): 16-byte rows, distinct bank groups
 constexpr int NW = H + H * IN + H + H * H + H + H * H + OUT + OUT * H;  // 51,104 floats in the file
-constexpr uint32_t LTWS_MAGIC = 0x4C545753u;
 
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
@@ -225,54 +225,27 @@ struct SuperResUpscaler {
 namespace {
 
 // All floats of an LTWS file, tensors concatenated in file order
-// (superres.c:43-101); NULL on a missing file or a foreign magic.
+// (superres.c:43-101, read by ltws.c); NULL on a missing, foreign or
+// truncated file.
 float *read_ltws(const char *path, long *count) {
-    FILE *f = fopen(path, "rb");
-    if (!f) {
+    LBMLtws t;
+    const int rc = lbm_ltws_read(path, &t);
+    if (rc == LBM_LTWS_CANNOT_OPEN) {
         printf("SR: cannot open weights file: %s\n", path);
         return NULL;
     }
-    uint32_t hdr[3];
-    if (fread(hdr, 4, 3, f) != 3 || hdr[0] != LTWS_MAGIC) {
+    if (rc == LBM_LTWS_BAD_MAGIC) {
         printf("SR: bad magic in %s\n", path);
-        fclose(f);
         return NULL;
     }
-    long total = 0, cap = 0;
-    float *data = NULL;
-    int ok = 1;
-    for (uint32_t t = 0; t < hdr[2] && ok; t++) {
-        uint32_t ndim = 0;
-        ok = fread(&ndim, 4, 1, f) == 1 && ndim <= 8;
-        long numel = 1;
-        for (uint32_t d = 0; d < ndim && ok; d++) {
-            uint32_t s;
-            ok = fread(&s, 4, 1, f) == 1;
-            numel *= (long)s;
-        }
-        if (!ok || numel < 0 || numel > (1L << 28)) {
-            ok = 0;
-            break;
-        }
-        if (total + numel > cap) {
-            cap = 2 * (total + numel);
-            float *grown = (float *)realloc(data, (size_t)cap * sizeof(float));
-            if (!grown) {
-                ok = 0;
-                break;
-            }
-            data = grown;
-        }
-        ok = fread(data + total, sizeof(float), (size_t)numel, f) == (size_t)numel;
-        total += numel;
-    }
-    fclose(f);
-    if (!ok) {
+    if (rc != LBM_LTWS_OK) {
         printf("SR: truncated weights file: %s\n", path);
-        free(data);
         return NULL;
     }
-    *count = total;
+    float *data = t.data ? t.data : (float *)malloc(sizeof(float));  // a file of no tensors holds 0 floats
+    *count = t.total;
+    t.data = NULL;
+    lbm_ltws_free(&t);
     return data;
 }
 

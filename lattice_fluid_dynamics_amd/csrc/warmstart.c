@@ -13,12 +13,10 @@
 
 #include "../../include/lbm_warmstart.h"
 #include "lbm_device.h"
+#include "ltws.h"
 
 /* lbm.c: the host copy of a grid's solid mask (sizeX * sizeY * sizeZ) */
 const int32_t *lbm_host_solid(const LBMGrid *g);
-
-#define LTWS_MAGIC 0x4C545753u
-#define LTWS_VERSION 1u
 
 enum { C0A, C0B, C1A, C1B, C2A, C2B, C3A, C3B, R3, D2, R2, D1, R1, D0, HEAD, NCONV };
 static const int CIN[NCONV] = {32, 32, 32, 64, 64, 128, 128, 256, 256, 128, 128, 64, 64, 32, 32};
@@ -412,25 +410,23 @@ int LBM_WarmStartSave(const LBMWarmStart *ws, const char *path) {
     if (!ws || !path) return 0;
     const long long n = LBM_WarmStartParamCount(ws);
     float *h = (float *)malloc(sizeof(float) * (size_t)n);
-    FILE *f = h ? fopen(path, "wb") : NULL;
-    if (!h || !f || !LBM_WarmStartGetParams(ws, 0, h)) {
-        if (f) fclose(f);
+    if (!h || !LBM_WarmStartGetParams(ws, 0, h)) {
         free(h);
         ws_err("LBM_WarmStartSave: cannot write the weights");
         return 0;
     }
-    uint32_t hdr[3] = {LTWS_MAGIC, LTWS_VERSION, (uint32_t)(2 * NCONV)};
-    int ok = fwrite(hdr, 4, 3, f) == 3;
-    long long o = 0;
-    for (int k = 0; k < NCONV && ok; k++) {
-        uint32_t wd[6] = {5, (uint32_t)COUT[k], 3, 3, 3, (uint32_t)CIN[k]};
-        uint32_t bd[2] = {1, (uint32_t)COUT[k]};
-        ok = fwrite(wd, 4, 6, f) == 6 && fwrite(h + o, 4, (size_t)ws->conv[k].nw, f) == (size_t)ws->conv[k].nw;
-        o += ws->conv[k].nw;
-        ok = ok && fwrite(bd, 4, 2, f) == 2 && fwrite(h + o, 4, (size_t)COUT[k], f) == (size_t)COUT[k];
-        o += COUT[k];
+    /* per convolution its weight (cout, 3, 3, 3, cin), then its bias (cout) */
+    uint32_t ndim[2 * NCONV], dims[2 * NCONV][LBM_LTWS_MAX_DIMS];
+    memset(dims, 0, sizeof dims);
+    for (int k = 0; k < NCONV; k++) {
+        const uint32_t wd[5] = {(uint32_t)COUT[k], 3, 3, 3, (uint32_t)CIN[k]};
+        ndim[2 * k] = 5;
+        memcpy(dims[2 * k], wd, sizeof wd);
+        ndim[2 * k + 1] = 1;
+        dims[2 * k + 1][0] = (uint32_t)COUT[k];
     }
-    ok = (fclose(f) == 0) && ok;
+    const int ok = lbm_ltws_write(path, 2 * NCONV, ndim, (const uint32_t(*)[LBM_LTWS_MAX_DIMS])dims, h) ==
+                   LBM_LTWS_OK;
     free(h);
     if (!ok) ws_err("LBM_WarmStartSave: write error");
     return ok;
@@ -438,34 +434,21 @@ int LBM_WarmStartSave(const LBMWarmStart *ws, const char *path) {
 
 int LBM_WarmStartLoad(LBMWarmStart *ws, const char *path) {
     if (!ws || !path) return 0;
-    FILE *f = fopen(path, "rb");
-    if (!f) {
+    LBMLtws t;
+    const int rc = lbm_ltws_read(path, &t);
+    if (rc == LBM_LTWS_CANNOT_OPEN) {
         ws_err("LBM_WarmStartLoad: cannot open the weights file");
         return 0;
     }
-    const long long n = LBM_WarmStartParamCount(ws);
-    float *h = (float *)malloc(sizeof(float) * (size_t)n);
-    uint32_t hdr[3];
-    int ok = h && fread(hdr, 4, 3, f) == 3 && hdr[0] == LTWS_MAGIC && hdr[1] == LTWS_VERSION &&
-             hdr[2] == (uint32_t)(2 * NCONV);
-    long long o = 0;
-    for (int p = 0; p < 2 * NCONV && ok; p++) {
-        const int k = p / 2;
-        uint32_t nd = 0, dims[8];
-        ok = fread(&nd, 4, 1, f) == 1 && nd >= 1 && nd <= 8 && fread(dims, 4, nd, f) == nd;
-        if (!ok) break;
+    int ok = rc == LBM_LTWS_OK && t.version == LBM_LTWS_VERSION && t.tensors == (uint32_t)(2 * NCONV) &&
+             t.total == LBM_WarmStartParamCount(ws);
+    for (int k = 0; k < NCONV && ok; k++) {
         const uint32_t want5[5] = {(uint32_t)COUT[k], 3, 3, 3, (uint32_t)CIN[k]};
-        if (p % 2 == 0)
-            ok = nd == 5 && memcmp(dims, want5, sizeof(want5)) == 0;
-        else
-            ok = nd == 1 && dims[0] == (uint32_t)COUT[k];
-        const size_t cnt = p % 2 == 0 ? (size_t)ws->conv[k].nw : (size_t)COUT[k];
-        ok = ok && fread(h + o, 4, cnt, f) == cnt;
-        o += (long long)cnt;
+        ok = t.ndim[2 * k] == 5 && memcmp(t.dims[2 * k], want5, sizeof want5) == 0 && t.ndim[2 * k + 1] == 1 &&
+             t.dims[2 * k + 1][0] == (uint32_t)COUT[k];
     }
-    fclose(f);
-    ok = ok && o == n && LBM_WarmStartSetParams(ws, h);
-    free(h);
+    ok = ok && LBM_WarmStartSetParams(ws, t.data);
+    lbm_ltws_free(&t);
     if (!ok) ws_err("LBM_WarmStartLoad: not an LTWS file of this network");
     return ok;
 }

@@ -18,7 +18,9 @@ from typing import Optional, Tuple
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblbm_amd.so")
+# LBM_AMD_LIBRARY names another build of the same library (the sanitizer
+# build of csrc/sanitize.mk)
+LIB_PATH = os.environ.get("LBM_AMD_LIBRARY") or os.path.join(_HERE, "liblbm_amd.so")
 
 LBM_STORAGE_FP32 = 0
 LBM_STORAGE_FP16 = 1

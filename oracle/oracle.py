@@ -29,7 +29,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liblbm_oracle.so")
+# LBM_ORACLE_LIBRARY names another build of the oracle (the sanitizer build)
+_LIB_PATH = os.environ.get("LBM_ORACLE_LIBRARY") or os.path.join(_HERE, "liblbm_oracle.so")
 _lib: Optional[ctypes.CDLL] = None
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
