@@ -7,14 +7,15 @@
 //                   candidates (cells outside the model box stay fluid);
 //   k_mesh_linkflag one lane per cell: fluid (solid != 1) with at least one
 //                   in-range non-fluid neighbour;
-//   compaction      hipCUB DeviceSelect::Flagged keeps those cells in cell
-//                   order (the order of the host's q rows);
+//   compaction      k_flag_count + k_flag_scatter keep those cells in cell
+//                   order (the order of the host's q rows): per tile of 4096
+//                   cells a count, an exclusive scan of the counts on the
+//                   host, then wave ballots place each flagged cell;
 //   k_mesh_q        one lane per (listed cell, direction): nearest hit of the
 //                   link over the same candidates, or -1 for no link.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -110,6 +111,48 @@ __global__ __launch_bounds__(256) void k_mesh_q(MeshDev m, Scale s, const int32_
     qrows[idx] = q;
 }
 
+// Order-preserving compaction of a byte flag array into cell indices.  A
+// workgroup owns one tile of CMP_TILE consecutive cells, visited in
+// CMP_ROUNDS rounds of 256; within a round, cells are placed by wave ballot
+// (lanes in index order) after the earlier waves of the round.
+constexpr int CMP_ROUNDS = 16, CMP_TILE = 256 * CMP_ROUNDS;
+
+__global__ __launch_bounds__(256) void k_flag_count(const uint8_t *__restrict__ flag, long long n,
+                                                    int *__restrict__ tileCount) {
+    __shared__ int total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * CMP_TILE;
+    int c = 0;
+    for (int r = 0; r < CMP_ROUNDS; r++) {
+        const long long i = base + r * 256 + threadIdx.x;
+        c += (i < n && flag[i]) ? 1 : 0;
+    }
+    if (c) atomicAdd(&total, c);
+    __syncthreads();
+    if (threadIdx.x == 0) tileCount[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_flag_scatter(const uint8_t *__restrict__ flag, long long n,
+                                                      const int *__restrict__ tileStart, int32_t *__restrict__ list) {
+    __shared__ int waveCount[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long base = (long long)blockIdx.x * CMP_TILE;
+    int out = tileStart[blockIdx.x];
+    for (int r = 0; r < CMP_ROUNDS; r++) {
+        const long long i = base + r * 256 + threadIdx.x;
+        const bool f = i < n && flag[i];
+        const unsigned long long m = __ballot(f);
+        if (lane == 0) waveCount[wave] = __popcll(m);
+        __syncthreads();
+        int off = out;
+        for (int w = 0; w < wave; w++) off += waveCount[w];
+        if (f) list[off + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
+        out += waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
+        __syncthreads();
+    }
+}
+
 struct Buf {
     void *p = nullptr;
     ~Buf() {
@@ -129,7 +172,7 @@ extern "C" int lbmdev_mesh_voxelize(int nx, int ny, int nz, const MeshIndex *m, 
     const long long ncell = (long long)nx * ny * nz;
     const int nbins = m->by * m->bz;
     const long long nlist = m->ntri > 0 ? m->binStart[nbins] : 0;
-    Buf tri, bounds, bstart, blist, solid, flag, list, count, temp, q;
+    Buf tri, bounds, bstart, blist, solid, flag, list, tiles, q;
 #define MV_CHECK(call)                                                                          \
     do {                                                                                        \
         hipError_t e_ = (call);                                                                 \
@@ -165,7 +208,6 @@ extern "C" int lbmdev_mesh_voxelize(int nx, int ny, int nz, const MeshIndex *m, 
     MV_CHECK(hipMalloc(&solid.p, sizeof(int32_t) * (size_t)ncell));
     MV_CHECK(hipMalloc(&flag.p, (size_t)ncell));
     MV_CHECK(hipMalloc(&list.p, sizeof(int32_t) * (size_t)ncell));
-    MV_CHECK(hipMalloc(&count.p, sizeof(int)));
     hipLaunchKernelGGL(k_mesh_solid, dim3(grid_of(ncell)), dim3(256), 0, 0, md, s, nx, ny, ncell, box[0], box[1],
                        box[2], box[3], box[4], box[5], (int32_t *)solid.p);
     MV_CHECK(hipGetLastError());
@@ -173,15 +215,30 @@ extern "C" int lbmdev_mesh_voxelize(int nx, int ny, int nz, const MeshIndex *m, 
                        ncell, (uint8_t *)flag.p);
     MV_CHECK(hipGetLastError());
     // order-preserving compaction of the flagged cell indices
-    hipcub::CountingInputIterator<int32_t> idx(0);
-    size_t tbytes = 0;
-    MV_CHECK(hipcub::DeviceSelect::Flagged(NULL, tbytes, idx, (const uint8_t *)flag.p, (int32_t *)list.p,
-                                           (int *)count.p, (int)ncell));
-    MV_CHECK(hipMalloc(&temp.p, tbytes > 0 ? tbytes : 1));
-    MV_CHECK(hipcub::DeviceSelect::Flagged(temp.p, tbytes, idx, (const uint8_t *)flag.p, (int32_t *)list.p,
-                                           (int *)count.p, (int)ncell));
-    int n = 0;
-    MV_CHECK(hipMemcpy(&n, count.p, sizeof(int), hipMemcpyDeviceToHost));
+    const long long ntile = (ncell + CMP_TILE - 1) / CMP_TILE;
+    MV_CHECK(hipMalloc(&tiles.p, sizeof(int) * (size_t)ntile));
+    hipLaunchKernelGGL(k_flag_count, dim3((unsigned)ntile), dim3(256), 0, 0, (const uint8_t *)flag.p, ncell,
+                       (int *)tiles.p);
+    MV_CHECK(hipGetLastError());
+    int *tileStart = (int *)malloc(sizeof(int) * (size_t)ntile);
+    if (!tileStart) return 0;
+    hipError_t ec = hipMemcpy(tileStart, tiles.p, sizeof(int) * (size_t)ntile, hipMemcpyDeviceToHost);
+    long long n64 = 0;
+    for (long long t = 0; t < ntile && ec == hipSuccess; t++) {  // exclusive scan (cell indices are int32)
+        const int c = tileStart[t];
+        tileStart[t] = (int)n64;
+        n64 += c;
+    }
+    if (ec == hipSuccess) ec = hipMemcpy(tiles.p, tileStart, sizeof(int) * (size_t)ntile, hipMemcpyHostToDevice);
+    free(tileStart);
+    MV_CHECK(ec);
+    if (n64 > 0x7fffffffLL) return 0;
+    const int n = (int)n64;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_flag_scatter, dim3((unsigned)ntile), dim3(256), 0, 0, (const uint8_t *)flag.p, ncell,
+                           (const int *)tiles.p, (int32_t *)list.p);
+        MV_CHECK(hipGetLastError());
+    }
     if (n > 0) {
         MV_CHECK(hipMalloc(&q.p, sizeof(float) * 19 * (size_t)n));
         hipLaunchKernelGGL(k_mesh_q, dim3(grid_of(19LL * n)), dim3(256), 0, 0, md, s, (const int32_t *)solid.p, nx,
