@@ -58,6 +58,8 @@ WORKLOADS = {
         dims=(256, 128, 128), op="mrt", smag=1, tau=0.52, u=0.06, particles=0, storage=1),
     "512x256x256-regularized-smagorinsky-fp16": dict(
         dims=(512, 256, 256), op="reg", smag=1, tau=0.505, u=0.05, particles=50000, storage=1),
+    "512x256x256-mrt-smagorinsky-fp16": dict(
+        dims=(512, 256, 256), op="mrt", smag=1, tau=0.505, u=0.05, particles=50000, storage=1),
 }
 DEFAULT = "512x256x256-mrt-smagorinsky"
 SECONDARY = "roofline-512x256x256-regularized-smagorinsky"

@@ -139,7 +139,7 @@ def _steps_to_1pct(h, ref, window):
 def learned_gain(dims=(128, 64, 64), factor: int = 4, n_train: int = 64, data_steps: int = 4000,
                  avg_steps: int = 1000, iters: int = 600, batch: int = 16, lr: float = 1e-3, n_eval: int = 8,
                  ref_steps: int = 8000, cold_steps: int = 8000, warm_steps: int = 4000, interval: int = 100,
-                 window: int = 5, tail: int = 20, log=None):
+                 window: int = 5, tail: int = 20, chunk: int = 256, log=None):
     """What the trained warm start gains over the cold start, end to end on
     the GPU: the FP32 solver generates the training fields (cold starts of
     the training tunnels, block-averaged to the coarse lattice and averaged
@@ -161,7 +161,8 @@ def learned_gain(dims=(128, 64, 64), factor: int = 4, n_train: int = 64, data_st
     nx, ny, nz = dims
     t0 = time.time()
     specs = WS.random_specs(range(n_train))
-    masks, fields = WS.solve_fields(specs, dims, factor, data_steps, avg_steps, interval, storage=0, chunk=n_train)
+    masks, fields = WS.solve_fields(specs, dims, factor, data_steps, avg_steps, interval, storage=0,
+                                    chunk=min(n_train, chunk), log=log)
     gen_s = time.time() - t0
     ws = WS.WarmStart(nx // factor, ny // factor, nz // factor, max_batch=batch, seed=0)
     data = WS.DeviceDataset(masks, fields, specs, augment=True)
@@ -173,9 +174,9 @@ def learned_gain(dims=(128, 64, 64), factor: int = 4, n_train: int = 64, data_st
         log(f"learned gain: {n_train} training tunnels solved in {gen_s:.1f}s, {iters} AdamW steps in "
             f"{train_s:.1f}s (loss {np.mean(losses[:20]):.4f} -> {np.mean(losses[-20:]):.5f})")
     ev = WS.random_specs(range(100000, 100000 + n_eval))
-    ref = WS.cd_histories(ev, dims, ref_steps, 3, None, factor, interval)
-    cold = WS.cd_histories(ev, dims, cold_steps, 0, None, factor, interval)
-    warm = WS.cd_histories(ev, dims, warm_steps, 1, ws, factor, interval)
+    ref = WS.cd_histories(ev, dims, ref_steps, 3, None, factor, interval, log=log)
+    cold = WS.cd_histories(ev, dims, cold_steps, 0, None, factor, interval, log=log)
+    warm = WS.cd_histories(ev, dims, warm_steps, 1, ws, factor, interval, log=log)
     ws.free()
     rows = []
     for t, sp in enumerate(ev):
@@ -205,6 +206,32 @@ def learned_gain(dims=(128, 64, 64), factor: int = 4, n_train: int = 64, data_st
                        "avg_steps": avg_steps, "iters": iters, "batch": batch, "n_eval": n_eval,
                        "ref_steps": ref_steps, "cold_steps": cold_steps, "warm_steps": warm_steps,
                        "interval": interval, "window": window, "tail": tail}}
+
+
+def gain_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 4000, steps: int = 8000,
+                    warm_steps: int = 4000, dims=(128, 64, 64)):
+    """learned_gain at the learned-start configuration's counts (e.g. 2000
+    training and 200 held-out tunnels) at half resolution: one JSON line per
+    held-out tunnel and a summary line."""
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    r = learned_gain(dims=dims, n_train=n_train, n_eval=n_eval, iters=iters, data_steps=data_steps,
+                     ref_steps=steps, cold_steps=steps, warm_steps=warm_steps, log=log)
+    for row in r["rows"]:
+        _emit({"experiment": "gain-tunnel", **row})
+    sp = [row["speedup"] for row in r["rows"] if row["speedup"] is not None]
+    _emit({"experiment": "gain-summary", "train_tunnels": n_train, "eval_tunnels": n_eval,
+           "lattice": "%dx%dx%d" % tuple(dims), "converged": r["converged"], "median_steps": r["median_steps"],
+           "speedup_median": r["median_speedup"], "speedup_geomean": r["geomean_speedup"],
+           "speedup_min": r["min_speedup"],
+           "speedup_p10_p90": [float(np.percentile(sp, 10)), float(np.percentile(sp, 90))] if sp else None,
+           "share_at_least_3x": float(np.mean(np.array(sp) >= 3.0)) if sp else None,
+           "ms_per_step_per_tunnel": r["ms_per_step_per_tunnel"], "warm_init_s_per_tunnel": r["warm_init_s_per_tunnel"],
+           "data_gen_s": round(r["data_gen_s"], 1), "train_s": round(r["train_s"], 1),
+           "loss_first": r["loss_first"], "loss_last": r["loss_last"], "params": r["params"],
+           "wall_s": round(time.time() - t0, 1)})
 
 
 def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 6000, avg_steps: int = 2000,
@@ -303,16 +330,16 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["storage", "sweep", "warmstart"])
+    ap.add_argument("which", choices=["storage", "sweep", "warmstart", "gain"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--tunnels", type=int, default=64)
     ap.add_argument("--storage", type=int, default=1)
     ap.add_argument("--train", type=int, default=384, help="warmstart: training tunnels")
     ap.add_argument("--eval", type=int, default=32, help="warmstart: held-out tunnels")
     ap.add_argument("--iters", type=int, default=3000, help="warmstart: training iterations")
-    ap.add_argument("--data-steps", type=int, default=6000)
-    ap.add_argument("--cold-steps", type=int, default=20000)
-    ap.add_argument("--warm-steps", type=int, default=12000)
+    ap.add_argument("--data-steps", type=int, default=None, help="warmstart: 6000; gain: 4000")
+    ap.add_argument("--cold-steps", type=int, default=None, help="warmstart: 20000; gain: 8000 (also the FP64 run)")
+    ap.add_argument("--warm-steps", type=int, default=None, help="warmstart: 12000; gain: 4000")
     ap.add_argument("--weights-in", default=None)
     ap.add_argument("--weights-out", default=None)
     ap.add_argument("--phase", choices=["all", "train", "eval"], default="all")
@@ -322,10 +349,14 @@ def main(argv=None):
         storage_experiment(a.steps or 20000)
     elif a.which == "sweep":
         sweep_experiment(a.tunnels, a.steps or 5000, a.storage)
+    elif a.which == "gain":
+        gain_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps or 4000, steps=a.cold_steps or 8000,
+                        warm_steps=a.warm_steps or 4000)
     else:
-        warmstart_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps, cold_steps=a.cold_steps,
-                             warm_steps=a.warm_steps, weights_in=a.weights_in, weights_out=a.weights_out,
-                             phase=a.phase, warm_fp32=not a.no_warm_fp32)
+        warmstart_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps or 6000,
+                             cold_steps=a.cold_steps or 20000, warm_steps=a.warm_steps or 12000,
+                             weights_in=a.weights_in, weights_out=a.weights_out, phase=a.phase,
+                             warm_fp32=not a.no_warm_fp32)
 
 
 if __name__ == "__main__":
