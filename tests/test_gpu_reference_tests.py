@@ -59,21 +59,23 @@ def test_mrt_smagorinsky_step(gpu):  # test_lbm.c:388-401
 def test_sphere_cd_re100(gpu):  # test_lbm.c:403-456
     """128x64x64, sphere of diameter 1 world unit = 16 cells at the origin,
     U 0.05, nu = U D / 100 (tau 0.524), BGK, clamped y/z (LBM_Create's
-    defaults), projected area taken before stepping, 4000 steps.  The
-    reference asserts 0.5 < Cd < 3.5 and observes Cd ~ 2.1.
+    defaults), projected area taken before stepping.  The reference asserts
+    0.5 < Cd < 3.5 after 4000 steps, as a guard against gross regressions
+    (Cd blowing up or collapsing to 0), and observes Cd ~ 2.1.
 
     The reference scheme itself loses stability in this configuration: in
     the FP32 oracle (C restatement of the shaders) max |u| grows from 0.058
     at step 2000 to 0.10 at step 2500, starting at the outlet corner cell
     (127, 63, 63) where the Zou-He pressure outlet meets the clamped y and z
     faces, and from step ~3000 the stability guard resets cells and Cd
-    wanders (oracle Cd at step 4000: 0.73).  Rounding differences grow with
-    it (FP32 and FP64 storage differ by 3e-3 in Cd at step 2000).  So the
-    test pins the flow before that point -- at step 1000, FP32 storage
-    against FP64 storage within 0.1 % and no guard reset in either -- and
-    keeps the reference's own
-    bound at step 4000 in FP32, the reference's precision; FP64's Cd and the
-    guard counts at step 4000 are printed."""
+    wanders.  From then on Cd at a given step depends on rounding: at step
+    4000 it is 0.73 in the FP32 oracle, 13.3 in FP64, and 0.4-2.6 in FP32
+    storage on the GPU depending on the rounding of its reciprocal and
+    square root.  So the test pins the flow while the scheme is stable: at
+    step 1000, FP32 storage against FP64 storage within 0.1 % and no guard
+    reset in either, and the reference's bound 0.5 < Cd < 3.5 at step 2000
+    in both precisions (1.25 flow-throughs; FP32 and FP64 within 1 % there).
+    Cd and the guard counts at step 4000 are printed."""
     U, D = 0.05, 1.0
     char_len = D * (128 / 8.0)
     visc = np.float32(U * char_len / 100.0)
@@ -85,19 +87,23 @@ def test_sphere_cd_re100(gpu):  # test_lbm.c:403-456
         gpu.LBM_SetSolidSphere(g, 0.0, 0.0, 0.0, D / 2)
         A = gpu.LBM_ComputeProjectedArea(g, 0)
         assert gpu.LBM_InitializeFlow(g, U, 0, 0) == 1
-        gpu.LBM_StepN(g, 1000, U, 0, 0)
-        early = (gpu.LBM_ComputeDragCoefficient(g, U, A), gpu.LBM_GetGuardResetCount(g))
-        gpu.LBM_StepN(g, 3000, U, 0, 0)
-        res[storage] = (early, (gpu.LBM_ComputeDragCoefficient(g, U, A), gpu.LBM_GetGuardResetCount(g)), A)
-    (cd1k, r1k), (cd, resets), A = res[0]
-    (cd1k64, r1k64), (cd64, resets64), _ = res[gpu.LBM_STORAGE_FP64]
+        out = []
+        for n in (1000, 1000, 2000):
+            gpu.LBM_StepN(g, n, U, 0, 0)
+            out.append((gpu.LBM_ComputeDragCoefficient(g, U, A), gpu.LBM_GetGuardResetCount(g)))
+        res[storage] = (out, A)
+    (c1, c2, c4), A = res[0]
+    (d1, d2, d4), _ = res[gpu.LBM_STORAGE_FP64]
     print("sphere Re=100: area %.0f; step 1000: Cd %.5f (FP32), %.5f (FP64), rel diff %.2e, guard resets %d / %d; "
+          "step 2000: Cd %.4f (FP32), %.4f (FP64), guard resets %d / %d; "
           "step 4000: Cd %.4f (FP32), %.4f (FP64), guard resets %d / %d"
-          % (A, cd1k, cd1k64, abs(cd1k - cd1k64) / cd1k64, r1k, r1k64, cd, cd64, resets, resets64))
+          % (A, c1[0], d1[0], abs(c1[0] - d1[0]) / d1[0], c1[1], d1[1], c2[0], d2[0], c2[1], d2[1],
+             c4[0], d4[0], c4[1], d4[1]))
     assert A > 150  # a 16-cell disc: ~ pi 8^2
-    assert r1k == 0 and r1k64 == 0
-    assert abs(cd1k - cd1k64) <= 1e-3 * cd1k64
-    assert 0.5 < cd < 3.5
+    assert c1[1] == 0 and d1[1] == 0
+    assert abs(c1[0] - d1[0]) <= 1e-3 * d1[0]
+    assert 0.5 < c2[0] < 3.5 and 0.5 < d2[0] < 3.5
+    assert abs(c2[0] - d2[0]) <= 1e-2 * d2[0]
 
 
 def test_box_cd_convergence(gpu):  # test_lbm.c:458-541
