@@ -368,12 +368,37 @@ __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollidePa
         f[12] = f[13] - (T(1) / T(6)) * uo;
     }
     // moments (lbm_collide.comp:411-424)
-    rho = T(0);
+    // MRT works through the sums sp and differences dp of the nine opposite
+    // pairs (1,2) (3,4) (5,6) (7,10) (8,9) (11,14) (12,13) (15,18) (16,17):
+    // every even moment is a sum of sp, every odd moment one of dp, and the
+    // collision's correction splits the same way (below).
+    T jx, jy, jz;
+    T sp[10], m16 = 0, m17 = 0, m18 = 0;
+    if constexpr (OP == OP_MRT) {
+        constexpr int PA[10] = {0, 1, 3, 5, 7, 8, 11, 12, 15, 16};
+        T dp[10];
 #pragma unroll
-    for (int i = 0; i < 19; i++) rho += f[i];
-    T jx = (f[1] - f[2]) + (f[7] - f[8]) + (f[9] - f[10]) + (f[11] - f[12]) + (f[13] - f[14]);
-    T jy = (f[3] - f[4]) + (f[7] - f[9]) + (f[8] - f[10]) + (f[15] - f[16]) + (f[17] - f[18]);
-    T jz = (f[5] - f[6]) + (f[11] - f[13]) + (f[12] - f[14]) + (f[15] - f[17]) + (f[16] - f[18]);
+        for (int q = 1; q < 10; q++) {
+            sp[q] = f[PA[q]] + f[OPP[PA[q]]];
+            dp[q] = f[PA[q]] - f[OPP[PA[q]]];
+        }
+        rho = f[0] + ((sp[1] + sp[2]) + sp[3]) + (((sp[4] + sp[5]) + (sp[6] + sp[7])) + (sp[8] + sp[9]));
+        const T t45 = dp[4] - dp[5], t67 = dp[6] - dp[7], u45 = dp[4] + dp[5], u67 = dp[6] + dp[7];
+        const T t89 = dp[8] - dp[9], u89 = dp[8] + dp[9];
+        jx = dp[1] + (t45 + t67);
+        jy = dp[2] + (u45 + t89);
+        jz = dp[3] + (u67 + u89);
+        m16 = t45 - t67;  // f7 - f8 + f9 - f10 - f11 + f12 - f13 + f14
+        m17 = t89 - u45;  // -f7 - f8 + f9 + f10 + f15 - f16 + f17 - f18
+        m18 = u67 - u89;  // f11 + f12 - f13 - f14 - f15 - f16 + f17 + f18
+    } else {
+        rho = T(0);
+#pragma unroll
+        for (int i = 0; i < 19; i++) rho += f[i];
+        jx = (f[1] - f[2]) + (f[7] - f[8]) + (f[9] - f[10]) + (f[11] - f[12]) + (f[13] - f[14]);
+        jy = (f[3] - f[4]) + (f[7] - f[9]) + (f[8] - f[10]) + (f[15] - f[16]) + (f[17] - f[18]);
+        jz = (f[5] - f[6]) + (f[11] - f[13]) + (f[12] - f[14]) + (f[15] - f[17]) + (f[16] - f[18]);
+    }
     if (rho > T(0.0001)) {
         T ir = trcp<T>(rho);
         ux = jx * ir;
@@ -398,7 +423,17 @@ __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollidePa
     const T c = T(1) - T(1.5) * u2;
     T tau_eff = (T)p.tau;
     T Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
-    if (p.smag || OP == OP_REG) {
+    if (OP == OP_MRT && p.smag) {
+        // as below, from the pair sums
+        const T sxy = sp[4] + sp[5], sxz = sp[6] + sp[7], syz = sp[8] + sp[9];
+        const T r3 = rho * (T(1) / T(3));
+        Pxx = (sp[1] + sxy + sxz) - (r3 + rho * ux * ux);
+        Pyy = (sp[2] + sxy + syz) - (r3 + rho * uy * uy);
+        Pzz = (sp[3] + sxz + syz) - (r3 + rho * uz * uz);
+        Pxy = (sp[4] - sp[5]) - rho * ux * uy;
+        Pxz = (sp[6] - sp[7]) - rho * ux * uz;
+        Pyz = (sp[8] - sp[9]) - rho * uy * uz;
+    } else if (p.smag || OP == OP_REG) {
         // non-equilibrium stress sum_i e_a e_b (f_i - feq_i), using the exact
         // D3Q19 identity sum_i e_a e_b feq_i = rho (delta_ab / 3 + u_a u_b)
         T sxy = f[7] + f[8] + f[9] + f[10];
@@ -436,91 +471,70 @@ __device__ __forceinline__ void collide_fluid(T f[19], T fo[19], const CollidePa
         for (int i = 0; i < 19; i++)
             fo[i] = feq_i<T>(i, rho, ux, uy, uz, c) + a * wgt<T>(i) * QP[i];
     } else if (OP == OP_MRT) {  // lbm_collide.comp:544-616
-        T m[19];
-        T sf = f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
-        T se = f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
-        m[0] = f[0] + sf + se;
-        m[1] = T(-30) * f[0] - T(11) * sf + T(8) * se;
-        m[2] = T(12) * f[0] - T(4) * sf + se;
-        T ax = (f[7] - f[8]) + (f[9] - f[10]) + (f[11] - f[12]) + (f[13] - f[14]);
-        T ay = (f[7] + f[8]) - (f[9] + f[10]) + (f[15] - f[16]) + (f[17] - f[18]);
-        T az = (f[11] + f[12]) - (f[13] + f[14]) + (f[15] + f[16]) - (f[17] + f[18]);
-        m[3] = (f[1] - f[2]) + ax;
-        m[4] = T(-4) * (f[1] - f[2]) + ax;
-        m[5] = (f[3] - f[4]) + ay;
-        m[6] = T(-4) * (f[3] - f[4]) + ay;
-        m[7] = (f[5] - f[6]) + az;
-        m[8] = T(-4) * (f[5] - f[6]) + az;
-        T sXY = f[7] + f[8] + f[9] + f[10];
-        T sXZ = f[11] + f[12] + f[13] + f[14];
-        T sYZ = f[15] + f[16] + f[17] + f[18];
-        T fx2 = f[1] + f[2], fy2 = f[3] + f[4], fz2 = f[5] + f[6];
-        m[9] = T(2) * fx2 - fy2 - fz2 + sXY + sXZ - T(2) * sYZ;
-        m[10] = T(-4) * fx2 + T(2) * fy2 + T(2) * fz2 + sXY + sXZ - T(2) * sYZ;
-        m[11] = fy2 - fz2 + sXY - sXZ;
-        m[12] = T(-2) * fy2 + T(2) * fz2 + sXY - sXZ;
-        m[13] = f[7] - f[8] - f[9] + f[10];
-        m[14] = f[15] - f[16] - f[17] + f[18];
-        m[15] = f[11] - f[12] - f[13] + f[14];
-        m[16] = f[7] - f[8] + f[9] - f[10] - f[11] + f[12] - f[13] + f[14];
-        m[17] = -f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
-        m[18] = f[11] + f[12] - f[13] - f[14] - f[15] - f[16] + f[17] + f[18];
+        // moments of the d'Humieres basis from the pair sums (m0, m3, m5, m7
+        // are rho and j; m16..m18 came with them)
+        const T sf = (sp[1] + sp[2]) + sp[3];
+        const T sXY = sp[4] + sp[5], sXZ = sp[6] + sp[7], sYZ = sp[8] + sp[9];
+        const T se = (sXY + sXZ) + sYZ;
+        const T m1 = T(-30) * f[0] - T(11) * sf + T(8) * se;
+        const T m2 = T(12) * f[0] - T(4) * sf + se;
+        const T m4 = jx - T(5) * (f[1] - f[2]);  // -4 (f1 - f2) + ax
+        const T m6 = jy - T(5) * (f[3] - f[4]);
+        const T m8 = jz - T(5) * (f[5] - f[6]);
+        const T A = T(2) * sp[1] - sp[2] - sp[3], B = (sXY + sXZ) - T(2) * sYZ;
+        const T m9 = A + B, m10 = B - T(2) * A;
+        const T C = sp[2] - sp[3], D = sXY - sXZ;
+        const T m11 = C + D, m12 = D - T(2) * C;
+        const T m13 = sp[4] - sp[5], m14 = sp[8] - sp[9], m15 = sp[6] - sp[7];
         // equilibrium moments (lbm_collide.comp:560-580)
-        T me[19];
-        me[0] = rho;
-        me[1] = T(-11) * rho + T(19) * rho * u2;
-        me[2] = T(3) * rho - T(5.5) * rho * u2;
-        me[3] = rho * ux;
-        me[4] = T(-2) / T(3) * rho * ux;
-        me[5] = rho * uy;
-        me[6] = T(-2) / T(3) * rho * uy;
-        me[7] = rho * uz;
-        me[8] = T(-2) / T(3) * rho * uz;
-        me[9] = rho * (T(2) * ux * ux - uy * uy - uz * uz);
-        me[10] = T(-0.5) * rho * (T(2) * ux * ux - uy * uy - uz * uz);
-        me[11] = rho * (uy * uy - uz * uz);
-        me[12] = T(-0.5) * rho * (uy * uy - uz * uz);
-        me[13] = rho * ux * uy;
-        me[14] = rho * uy * uz;
-        me[15] = rho * ux * uz;
-        me[16] = me[17] = me[18] = T(0);
-        // Relaxation rates (lbm_collide.comp:585-604).  f* = M^-1 (m - S (m - meq))
-        // is applied as f* = f - M^-1 S (m - meq): the conserved moments
-        // (rho, j; s = 0) drop out, so mass and momentum are conserved to
-        // unbiased rounding (scaling M^-1 by rounded reciprocals of the row
-        // norms would otherwise bias them by ~1e-8 per step).
-        const T s[19] = {T(0),   T(1.19), T(1.4), T(0),   T(1.2), T(0),   T(1.2), T(0),    T(1.2),   omega,
-                         T(1.4), omega,   T(1.4), omega,  omega,  omega,  T(1.98), T(1.98), T(1.98)};
-        // 1/|row|^2 of M (lbm_collide.comp:188-206)
-        const T inrm[19] = {T(1) / T(19), T(1) / T(2394), T(1) / T(252), T(1) / T(10), T(1) / T(40),
-                            T(1) / T(10), T(1) / T(40),   T(1) / T(10),  T(1) / T(40), T(1) / T(36),
-                            T(1) / T(72), T(1) / T(12),   T(1) / T(24),  T(0.25),      T(0.25),
-                            T(0.25),      T(0.125),       T(0.125),      T(0.125)};
-        T c[19];
-#pragma unroll
-        for (int k = 0; k < 19; k++) c[k] = s[k] * (m[k] - me[k]) * inrm[k];
-        const T b0 = -T(11) * c[1] - T(4) * c[2];
-        const T b1 = T(8) * c[1] + c[2];
-        const T p910 = c[9] + c[10], p1112 = c[11] + c[12];
-        fo[0] = f[0] - (-T(30) * c[1] + T(12) * c[2]);
-        fo[1] = f[1] - (b0 - T(4) * c[4] + T(2) * c[9] - T(4) * c[10]);
-        fo[2] = f[2] - (b0 + T(4) * c[4] + T(2) * c[9] - T(4) * c[10]);
-        fo[3] = f[3] - (b0 - T(4) * c[6] - c[9] + T(2) * c[10] + c[11] - T(2) * c[12]);
-        fo[4] = f[4] - (b0 + T(4) * c[6] - c[9] + T(2) * c[10] + c[11] - T(2) * c[12]);
-        fo[5] = f[5] - (b0 - T(4) * c[8] - c[9] + T(2) * c[10] - c[11] + T(2) * c[12]);
-        fo[6] = f[6] - (b0 + T(4) * c[8] - c[9] + T(2) * c[10] - c[11] + T(2) * c[12]);
-        fo[7] = f[7] - (b1 + c[4] + c[6] + p910 + p1112 + c[13] + c[16] - c[17]);
-        fo[8] = f[8] - (b1 - c[4] + c[6] + p910 + p1112 - c[13] - c[16] - c[17]);
-        fo[9] = f[9] - (b1 + c[4] - c[6] + p910 + p1112 - c[13] + c[16] + c[17]);
-        fo[10] = f[10] - (b1 - c[4] - c[6] + p910 + p1112 + c[13] - c[16] + c[17]);
-        fo[11] = f[11] - (b1 + c[4] + c[8] + p910 - p1112 + c[15] - c[16] + c[18]);
-        fo[12] = f[12] - (b1 - c[4] + c[8] + p910 - p1112 - c[15] + c[16] + c[18]);
-        fo[13] = f[13] - (b1 + c[4] - c[8] + p910 - p1112 - c[15] - c[16] - c[18]);
-        fo[14] = f[14] - (b1 - c[4] - c[8] + p910 - p1112 + c[15] + c[16] - c[18]);
-        fo[15] = f[15] - (b1 + c[6] + c[8] - T(2) * p910 + c[14] + c[17] - c[18]);
-        fo[16] = f[16] - (b1 - c[6] + c[8] - T(2) * p910 - c[14] - c[17] - c[18]);
-        fo[17] = f[17] - (b1 + c[6] - c[8] - T(2) * p910 - c[14] + c[17] + c[18]);
-        fo[18] = f[18] - (b1 - c[6] - c[8] - T(2) * p910 + c[14] - c[17] + c[18]);
+        const T e9 = rho * (T(2) * ux * ux - uy * uy - uz * uz), e11 = rho * (uy * uy - uz * uz);
+        // Relaxation rates (lbm_collide.comp:585-604) over the squared row
+        // norms of M (lbm_collide.comp:188-206): c_k = s_k (m_k - meq_k) / |M_k|^2.
+        // f* = M^-1 (m - S (m - meq)) is applied as f* = f - M^T c: the
+        // conserved moments (rho, j; s = 0) drop out, so mass and momentum
+        // are conserved to unbiased rounding.
+        const T c1 = (m1 - (T(-11) * rho + T(19) * rho * u2)) * T(1.19 / 2394.0);
+        const T c2 = (m2 - (T(3) * rho - T(5.5) * rho * u2)) * T(1.4 / 252.0);
+        const T c4 = (m4 - T(-2) / T(3) * jx) * T(1.2 / 40.0);
+        const T c6 = (m6 - T(-2) / T(3) * jy) * T(1.2 / 40.0);
+        const T c8 = (m8 - T(-2) / T(3) * jz) * T(1.2 / 40.0);
+        const T c9 = omega * (m9 - e9) * T(1.0 / 36.0);
+        const T c10 = (m10 - T(-0.5) * e9) * T(1.4 / 72.0);
+        const T c11 = omega * (m11 - e11) * T(1.0 / 12.0);
+        const T c12 = (m12 - T(-0.5) * e11) * T(1.4 / 24.0);
+        const T c13 = omega * (m13 - rho * ux * uy) * T(0.25);
+        const T c14 = omega * (m14 - rho * uy * uz) * T(0.25);
+        const T c15 = omega * (m15 - rho * ux * uz) * T(0.25);
+        const T c16 = m16 * T(1.98 * 0.125), c17 = m17 * T(1.98 * 0.125), c18 = m18 * T(1.98 * 0.125);
+        // M^T c per pair: the even part E (shared by both members) and the
+        // odd part O (+ for the first member, - for the second)
+        const T b0 = T(-11) * c1 - T(4) * c2, b1 = T(8) * c1 + c2;
+        const T p910 = c9 + c10, p1112 = c11 + c12;
+        const T t0 = b0 - c9 + T(2) * c10, u0 = c11 - T(2) * c12;
+        const T E1 = b0 + T(2) * c9 - T(4) * c10, E2 = t0 + u0, E3 = t0 - u0;
+        const T q0 = b1 + p910, r0 = q0 + p1112, r1 = q0 - p1112, v0 = b1 - T(2) * p910;
+        const T E4 = r0 + c13, E5 = r0 - c13, E6 = r1 + c15, E7 = r1 - c15, E8 = v0 + c14, E9 = v0 - c14;
+        const T g0 = c6 - c17, h0 = c4 + c16, k0 = c8 + c18, l0 = c4 - c16, n0 = c8 - c18, o0 = c6 + c17;
+        const T O4 = g0 + h0, O5 = g0 - h0, O6 = k0 + l0, O7 = k0 - l0, O8 = n0 + o0, O9 = n0 - o0;
+        fo[0] = f[0] - (T(-30) * c1 + T(12) * c2);
+        fo[1] = f[1] - (E1 - T(4) * c4);
+        fo[2] = f[2] - (E1 + T(4) * c4);
+        fo[3] = f[3] - (E2 - T(4) * c6);
+        fo[4] = f[4] - (E2 + T(4) * c6);
+        fo[5] = f[5] - (E3 - T(4) * c8);
+        fo[6] = f[6] - (E3 + T(4) * c8);
+        fo[7] = f[7] - (E4 + O4);
+        fo[10] = f[10] - (E4 - O4);
+        fo[8] = f[8] - (E5 + O5);
+        fo[9] = f[9] - (E5 - O5);
+        fo[11] = f[11] - (E6 + O6);
+        fo[14] = f[14] - (E6 - O6);
+        fo[12] = f[12] - (E7 + O7);
+        fo[13] = f[13] - (E7 - O7);
+        fo[15] = f[15] - (E8 + O8);
+        fo[18] = f[18] - (E8 - O8);
+        fo[16] = f[16] - (E9 + O9);
+        fo[17] = f[17] - (E9 - O9);
     } else {  // BGK, lbm_collide.comp:618-625
 #pragma unroll
         for (int i = 0; i < 19; i++) fo[i] = f[i] - omega * (f[i] - feq_i<T>(i, rho, ux, uy, uz, c));

@@ -21,7 +21,15 @@ error settles near the per-store rounding over the flow's population
 deviations: about 1e-3 of u (FP16S) and 1e-2 (BF16S) relative L2.  The
 bars below are about twice the values measured on MI355X (printed by the
 tests); BF16S's Cd is 9 % off after 20,000 steps, so it is a storage format
-for throughput runs, not for drag figures."""
+for throughput runs, not for drag figures.
+
+The velocity error of the 16-bit formats is bounded away from the outlet
+(x < nx - 8).  The Zou-He pressure outlet amplifies the storage rounding in
+its last few planes by an amount that varies from run to run: with inlet
+speeds 0.0599, 0.06, 0.0601 and 0.0602 the FP16S error over the whole
+lattice after 300 steps scattered over 3.0e-4 - 1.4e-3, while the error
+over x < nx - 8 stayed at 2.5e-4 - 3.3e-4 (BF16S 2.9e-3 - 4.0e-3).  The
+whole-lattice error is printed and held to a looser bar."""
 import numpy as np
 import pytest
 
@@ -49,11 +57,12 @@ def setup_mesh(tris, bbox):
 
 
 # stated per-format bars against the FP64 oracle after 300 steps at 128x64x64
-# (measured on MI355X: FP32 u 1.5e-6, rho 9.1e-8, Cd 2.0e-6; FP16S u 3.7e-4,
-# rho 1.4e-5, Cd 3.0e-3; BF16S u 3.1e-3, rho 1.9e-4, Cd 1.75e-2)
-BARS_300 = {0: {"u": 1e-5, "rho": 1e-6, "cd": 1e-4},
-            1: {"u": 8e-4, "rho": 3e-5, "cd": 6e-3},
-            2: {"u": 6e-3, "rho": 4e-4, "cd": 3.5e-2}}
+# (measured on MI355X: FP32 u 1.8e-6 (x < nx - 8: 9.1e-7), rho 6.7e-8, Cd 8.1e-6;
+# FP16S u 9.8e-4 (2.8e-4), rho 1.4e-5, Cd 3.2e-3; BF16S u 3.1e-3 (2.9e-3),
+# rho 1.9e-4, Cd 1.2e-2)
+BARS_300 = {0: {"u": 1e-5, "u_inner": 1e-5, "rho": 1e-6, "cd": 1e-4},
+            1: {"u": 3e-3, "u_inner": 6e-4, "rho": 3e-5, "cd": 6e-3},
+            2: {"u": 1e-2, "u_inner": 8e-3, "rho": 4e-4, "cd": 3.5e-2}}
 
 
 def test_config2_shape_storage_formats_against_the_oracles(gpu):
@@ -117,7 +126,9 @@ def test_config2_shape_storage_formats_against_the_oracles(gpu):
     for st in (0, 1, 2):
         vel = gpu.LBM_CopyVelocityToHost(grids[st])
         F = gpu.LBM_ComputeDragForceF64(grids[st])
-        err = {"u": rel_l2(vel[:, :3], ref_u), "rho": rel_l2(vel[:, 3], ref_rho),
+        inner = (np.arange(vel.shape[0]) % nx) < nx - 8
+        err = {"u": rel_l2(vel[:, :3], ref_u), "u_inner": rel_l2(vel[inner, :3], ref_u[inner]),
+               "rho": rel_l2(vel[:, 3], ref_rho),
                "cd": abs(abs(F[0]) / dyn - cd_ref) / cd_ref, "cl_abs": abs(F[1] / dyn - cl_ref),
                "cd_pressure": abs(abs(F[3]) / dyn - cdp_ref) / cdp_ref}
         print("%s vs FP64 oracle after %d steps: %s (Cd %.5f, FP64 %.5f; Cl FP64 %.5f)" % (
