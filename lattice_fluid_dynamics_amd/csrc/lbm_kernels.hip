@@ -2148,8 +2148,12 @@ extern "C" int lbmdev_profile_steps(LbmDev *d, const LbmStepParams *p, int nstep
     hipEvent_t *ev = NULL;
     const int per = (main_ms || pre_ms) ? 1 : 0;
     if (per) {
+        // Timing-only events: no system-scope fence (cache write-back and
+        // invalidation) when they are recorded, so they disturb the launches
+        // they bracket as little as possible.  t1 below keeps the fence and is
+        // what the host waits on.
         ev = (hipEvent_t *)calloc((size_t)nsteps * 3, sizeof(hipEvent_t));
-        for (int k = 0; k < 3 * nsteps; k++) hipEventCreate(&ev[k]);
+        for (int k = 0; k < 3 * nsteps; k++) hipEventCreateWithFlags(&ev[k], hipEventDisableSystemFence);
     }
     HIPCHECK(hipStreamSynchronize(d->stream), "synchronize");
     // Whether the first step had a prepass (only the first step after
