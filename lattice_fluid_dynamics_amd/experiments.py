@@ -237,7 +237,8 @@ def gain_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 400
 def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int = 6000, avg_steps: int = 2000,
                          cold_steps: int = 20000, warm_steps: int = 12000, batch: int = 16, lr: float = 1e-3,
                          weights_in=None, weights_out=None, chunk: int = 96, window: int = 20,
-                         dims=(256, 128, 128), factor: int = 4, phase: str = "all", warm_fp32: bool = True):
+                         dims=(256, 128, 128), factor: int = 4, phase: str = "all", warm_fp32: bool = True,
+                         reference: str = "fp32", eval_offset: int = 0):
     """Config 5: learned warm start at 256x128x128 (coarse lattice 64x32x32).
 
     Training tunnels use body seeds 0..n_train-1, held-out tunnels seeds
@@ -247,7 +248,10 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
     lattice and averaged over the next `avg_steps`.  Evaluation: Cd sampled
     every 100 steps; time to converged Cd = first step after which the mean
     of the last `window` samples stays within 1 % of the held-out tunnel's
-    FP32 cold-start converged Cd (mean of its last 6000 steps)."""
+    converged Cd: the mean of the last 6000 steps of its FP32 cold start, or
+    with reference="fp64" of its cold start in FP64 storage (the GPU path
+    that equals the FP64 oracle).  eval_offset shifts the held-out seeds, so
+    that one evaluation can be split over several runs."""
     from . import warmstart as WS
 
     def log(m):
@@ -280,8 +284,11 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
         del data
     if phase == "train":
         return
-    specs = WS.random_specs(range(100000, 100000 + n_eval))
+    specs = WS.random_specs(range(100000 + eval_offset, 100000 + eval_offset + n_eval))
     runs = {}
+    ref_run = None
+    if reference == "fp64":
+        ref_run = WS.cd_histories(specs, dims, cold_steps, 3, None, factor, log=log)
     runs["cold_fp32"] = WS.cd_histories(specs, dims, cold_steps, 0, None, factor, avg_from=cold_steps - 6000, log=log)
     runs["cold_fp16"] = WS.cd_histories(specs, dims, warm_steps, 1, None, factor, log=log)
     runs["warm_fp16"] = WS.cd_histories(specs, dims, warm_steps, 1, ws, factor, log=log)
@@ -290,8 +297,9 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
     conv = runs["cold_fp32"]["mean_field"]
     rows = []
     for t, sp in enumerate(specs):
-        ref = runs["cold_fp32"]["histories"][t].converged_value(tail=60)
-        row = {"seed": sp.seed, "u": round(sp.u, 5), "tau": round(sp.tau, 4), "cd_ref": round(ref, 5)}
+        ref = (ref_run or runs["cold_fp32"])["histories"][t].converged_value(tail=60)
+        row = {"seed": sp.seed, "u": round(sp.u, 5), "tau": round(sp.tau, 4), "cd_ref": round(ref, 5),
+               "reference": reference}
         for name, r in runs.items():
             st = _steps_to_1pct(r["histories"][t], ref, window)
             per_tunnel_ms = r["ms_per_step"] / len(specs)
@@ -316,7 +324,8 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
 
     ratios = [r["cold_fp32_s"] / r["warm_fp16_s"] for r in rows
               if r["cold_fp32_s"] is not None and r["warm_fp16_s"] is not None]
-    _emit({"experiment": "config5-summary", "eval_tunnels": n_eval, "window_samples": window,
+    _emit({"experiment": "config5-summary", "eval_tunnels": n_eval, "eval_seeds_from": 100000 + eval_offset,
+           "reference": reference, "window_samples": window,
            "sample_interval": 100,
            **{k + "_ms_per_step_per_tunnel": round(r["ms_per_step"] / n_eval, 5) for k, r in runs.items()},
            **{k + "_converged": sum(r2[k + "_steps"] is not None for r2 in rows) for k in runs},
@@ -344,6 +353,9 @@ def main(argv=None):
     ap.add_argument("--weights-out", default=None)
     ap.add_argument("--phase", choices=["all", "train", "eval"], default="all")
     ap.add_argument("--no-warm-fp32", action="store_true", help="warmstart: skip the warm-start FP32 runs")
+    ap.add_argument("--reference", choices=["fp32", "fp64"], default="fp32",
+                    help="warmstart: the cold start whose converged Cd is the target")
+    ap.add_argument("--eval-offset", type=int, default=0, help="warmstart: first held-out seed - 100000")
     a = ap.parse_args(argv)
     if a.which == "storage":
         storage_experiment(a.steps or 20000)
@@ -356,7 +368,7 @@ def main(argv=None):
         warmstart_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps or 6000,
                              cold_steps=a.cold_steps or 20000, warm_steps=a.warm_steps or 12000,
                              weights_in=a.weights_in, weights_out=a.weights_out, phase=a.phase,
-                             warm_fp32=not a.no_warm_fp32)
+                             warm_fp32=not a.no_warm_fp32, reference=a.reference, eval_offset=a.eval_offset)
 
 
 if __name__ == "__main__":
