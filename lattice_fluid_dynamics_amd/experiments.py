@@ -238,7 +238,8 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
                          cold_steps: int = 20000, warm_steps: int = 12000, batch: int = 16, lr: float = 1e-3,
                          weights_in=None, weights_out=None, chunk: int = 96, window: int = 20,
                          dims=(256, 128, 128), factor: int = 4, phase: str = "all", warm_fp32: bool = True,
-                         reference: str = "fp32", eval_offset: int = 0):
+                         reference: str = "fp32", eval_offset: int = 0, bootstrap: int = 0, boot_steps: int = 1200,
+                         boot_avg: int = 1200, boot_iters: int = 1500):
     """Config 5: learned warm start at 256x128x128 (coarse lattice 64x32x32).
 
     Training tunnels use body seeds 0..n_train-1, held-out tunnels seeds
@@ -251,7 +252,14 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
     converged Cd: the mean of the last 6000 steps of its FP32 cold start, or
     with reference="fp64" of its cold start in FP64 storage (the GPU path
     that equals the FP64 oracle).  eval_offset shifts the held-out seeds, so
-    that one evaluation can be split over several runs."""
+    that one evaluation can be split over several runs.
+
+    bootstrap = N > 0 generates the training fields in two stages, so that
+    many more tunnels fit one run: the first N tunnels from cold starts as
+    above, a network trained on them (boot_iters AdamW steps), then every
+    other training tunnel started from that network's prediction and solved
+    in FP32 for boot_steps steps, its field averaged over the next boot_avg.
+    The final network is trained from scratch on all the fields."""
     from . import warmstart as WS
 
     def log(m):
@@ -264,8 +272,18 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
     if phase in ("all", "train") and not weights_in:
         specs = WS.random_specs(range(n_train))
         t0 = time.time()
-        masks, fields = WS.solve_fields(specs, dims, factor, data_steps, avg_steps, 100, storage=0, chunk=chunk,
-                                        log=log)
+        n_cold = bootstrap if 0 < bootstrap < n_train else n_train
+        masks, fields = WS.solve_fields(specs[:n_cold], dims, factor, data_steps, avg_steps, 100, storage=0,
+                                        chunk=chunk, log=log)
+        if n_cold < n_train:
+            boot = WS.WarmStart(nx // factor, ny // factor, nz // factor, max_batch=batch, seed=1)
+            bdata = WS.DeviceDataset(masks, fields, specs[:n_cold], augment=True)
+            WS.train(boot, bdata, boot_iters, batch=batch, lr=lr, wd=1e-4, log=log)
+            del bdata
+            m2, f2 = WS.solve_fields(specs[n_cold:], dims, factor, boot_steps, boot_avg, 100, storage=0,
+                                     chunk=chunk, log=log, ws=boot)
+            boot.free()
+            masks, fields = np.concatenate([masks, m2]), np.concatenate([fields, f2])
         gen_s = time.time() - t0
         data = WS.DeviceDataset(masks, fields, specs, augment=True)
         t0 = time.time()
@@ -274,6 +292,8 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
         if weights_out:
             ws.save(weights_out)
         _emit({"experiment": "config5-train", "train_tunnels": n_train, "samples_with_mirrors": data.n,
+               "cold_data_tunnels": n_cold, "bootstrap": {"steps": boot_steps, "avg_steps": boot_avg,
+                                                          "iters": boot_iters} if n_cold < n_train else None,
                "lattice": f"{nx}x{ny}x{nz}", "coarse": f"{nx // factor}x{ny // factor}x{nz // factor}",
                "data_steps": data_steps, "avg_steps": avg_steps, "data_gen_s": round(gen_s, 1),
                "iters": iters, "batch": batch, "lr": lr, "train_s": round(train_s, 1),
@@ -356,6 +376,8 @@ def main(argv=None):
     ap.add_argument("--reference", choices=["fp32", "fp64"], default="fp32",
                     help="warmstart: the cold start whose converged Cd is the target")
     ap.add_argument("--eval-offset", type=int, default=0, help="warmstart: first held-out seed - 100000")
+    ap.add_argument("--bootstrap", type=int, default=0,
+                    help="warmstart: cold-start only this many training tunnels, warm-start the rest")
     a = ap.parse_args(argv)
     if a.which == "storage":
         storage_experiment(a.steps or 20000)
@@ -368,7 +390,8 @@ def main(argv=None):
         warmstart_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps or 6000,
                              cold_steps=a.cold_steps or 20000, warm_steps=a.warm_steps or 12000,
                              weights_in=a.weights_in, weights_out=a.weights_out, phase=a.phase,
-                             warm_fp32=not a.no_warm_fp32, reference=a.reference, eval_offset=a.eval_offset)
+                             warm_fp32=not a.no_warm_fp32, reference=a.reference, eval_offset=a.eval_offset,
+                             bootstrap=a.bootstrap)
 
 
 if __name__ == "__main__":

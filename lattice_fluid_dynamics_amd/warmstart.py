@@ -317,15 +317,20 @@ def build_batch(specs: Sequence[TunnelSpec], dims, factor: int, storage: int):
 
 
 def solve_fields(specs: Sequence[TunnelSpec], dims=(256, 128, 128), factor: int = 4, steps: int = 8000,
-                 avg_steps: int = 2000, avg_every: int = 100, storage: int = 0, chunk: int = 128, log=None):
-    """Cold-start solutions: masks (N, cz, cy, cx) and fields (N, cz, cy, cx, 4),
-    the block-averaged field averaged over `avg_steps` after `steps`."""
+                 avg_steps: int = 2000, avg_every: int = 100, storage: int = 0, chunk: int = 128, log=None,
+                 ws: Optional["WarmStart"] = None):
+    """Solutions: masks (N, cz, cy, cx) and fields (N, cz, cy, cx, 4), the
+    block-averaged field averaged over `avg_steps` after `steps`, from a cold
+    start, or from the prediction of `ws` when it is given."""
     masks, fields = [], []
     for c0 in range(0, len(specs), chunk):
         part = specs[c0:c0 + chunk]
         t0 = time.time()
         b, m, _ = build_batch(part, dims, factor, storage)
         b.initialize()
+        if ws is not None:
+            for t, sp in enumerate(part):
+                b.initialize_from_field(t, ws.predict(m[t], sp.u, sp.nu))
         t1 = time.time()
         if log:
             log(f"tunnels {c0}..{c0 + len(part) - 1}: set up in {t1 - t0:.1f}s")
