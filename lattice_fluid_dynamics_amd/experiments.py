@@ -363,6 +363,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--tunnels", type=int, default=64)
     ap.add_argument("--storage", type=int, default=1)
+    ap.add_argument("--tune-steps", type=int, default=20, help="sweep: steps per autotuner candidate")
     ap.add_argument("--train", type=int, default=384, help="warmstart: training tunnels")
     ap.add_argument("--eval", type=int, default=32, help="warmstart: held-out tunnels")
     ap.add_argument("--iters", type=int, default=3000, help="warmstart: training iterations")
@@ -382,7 +383,7 @@ def main(argv=None):
     if a.which == "storage":
         storage_experiment(a.steps or 20000)
     elif a.which == "sweep":
-        sweep_experiment(a.tunnels, a.steps or 5000, a.storage)
+        sweep_experiment(a.tunnels, a.steps or 5000, a.storage, tune_steps=a.tune_steps)
     elif a.which == "gain":
         gain_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps or 4000, steps=a.cold_steps or 8000,
                         warm_steps=a.warm_steps or 4000)
