@@ -25,9 +25,9 @@ per second of the whole timed region (inputs resident in HBM; includes the
 boundary prepass, the particle kernel and the host synchronisations).
 `roofline` prices the fused step kernel alone at 152 B/cell/step (19 loads +
 19 stores of FP32) against its average launch time from HIP events recorded on
-the grid's stream around every launch of every 10th frame of the timed region
-(an event pair stalls the queue for several microseconds, so the other frames
-run without them).  `cpu_baseline` times the oracle (the
+the grid's stream around every launch of every 10th frame of the timed region,
+counted back from its last frame (each such frame ends with a host wait, so
+the other frames run without events).  `cpu_baseline` times the oracle (the
 C/OpenMP restatement of the reference shaders) on a bounded sample of the same
 workload on the host cores.
 
@@ -99,10 +99,11 @@ def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
     """nsteps LBM steps in frames of 10 (particles advected after each frame).
     Frames are queued without host synchronisation (LBM_StepN), as the
     reference's main loop steps between samples.  With event_every = k > 0,
-    every k-th frame runs through LBM_ProfileSteps instead, whose launches
-    are bracketed by HIP events on the grid's stream and which synchronises
-    before and after (a recorded event pair idles the GPU for several
-    microseconds, so only a sample of the frames carries them).
+    every k-th frame counted back from the last runs through LBM_ProfileSteps
+    instead, whose launches are bracketed by HIP events on the grid's stream
+    and which waits for its last event (the wait of the last frame is the
+    region's closing synchronisation; only a sample of the frames carries
+    events, so the other waits stay rare).
     Returns (wall_s, main_ms_sum, boundary_ms_sum, timed_steps)."""
     nx = w["dims"][0]
     vel_scale = 8.0 / nx
@@ -112,9 +113,10 @@ def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
     t0 = time.perf_counter()
     done = 0
     frame = 0
+    frames = (nsteps + 9) // 10
     while done < nsteps:
         n = min(10, nsteps - done)
-        sample = event_every > 0 and frame % event_every == 0
+        sample = event_every > 0 and (frames - 1 - frame) % event_every == 0
         if sample:
             _, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=True)
             main += mn

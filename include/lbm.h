@@ -223,7 +223,9 @@ void LBM_Synchronize(LBMGrid *grid);
 
 /* Runs n steps and reports HIP-event times on the grid's stream: total_ms for
  * the whole region, main_ms and boundary_ms summed over the fused step kernel
- * and the boundary prepass launches (NULL to skip per-launch events). */
+ * and the boundary prepass launches (NULL to skip per-launch events).  Work
+ * queued before the call is not waited for (the events are stream-ordered);
+ * the call returns once its last launch has finished. */
 int LBM_ProfileSteps(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inletVelZ,
                      float *total_ms, float *main_ms, float *boundary_ms);
 

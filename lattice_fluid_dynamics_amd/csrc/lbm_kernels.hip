@@ -2155,9 +2155,10 @@ extern "C" int lbmdev_profile_steps(LbmDev *d, const LbmStepParams *p, int nstep
         ev = (hipEvent_t *)calloc((size_t)nsteps * 3, sizeof(hipEvent_t));
         for (int k = 0; k < 3 * nsteps; k++) hipEventCreateWithFlags(&ev[k], hipEventDisableSystemFence);
     }
-    HIPCHECK(hipStreamSynchronize(d->stream), "synchronize");
-    // Whether the first step had a prepass (only the first step after
-    // LBM_InitializeFlow has none).
+    // No host synchronisation before the region: t0 and the per-launch events
+    // are stream-ordered, so work queued earlier is not counted, and the GPU
+    // does not idle while the host waits.  Whether the first step had a
+    // prepass (only the first step after LBM_InitializeFlow has none).
     const int first_init = d->state == ST_INIT;
     hipEventRecord(t0, d->stream);
     int ok = dispatch_steps(d, p, nsteps, ev);
