@@ -642,21 +642,29 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
 #pragma unroll
         for (int i = 0; i < 19; i++)
             raw[i] = io.ld(OPP[i], (uint32_t)(X[1] + Y[1 - EY[i]] + Z[1 - EZ[i]]));
+        // One branch per edge lane for the LDS writes; the reads are
+        // uniform-address broadcasts taken by every lane and kept by a
+        // select, so no per-direction exec-mask branches.
+        if (lane == 63) {
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-            if (lane == 63) lp[w][k] = raw[XPLUS[k]];
-            if (lane == 0) lm[w][k] = raw[XMINUS[k]];
+            for (int k = 0; k < 5; k++) lp[w][k] = raw[XPLUS[k]];
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) lm[w][k] = raw[XMINUS[k]];
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 19; i++) {
             S v = raw[i];
             if (EX[i] == 1) {
+                const S e = lp[wprev][xdir_slot(i)];
                 v = wave_shift1<true>(v);
-                if (lane == 0) v = lp[wprev][xdir_slot(i)];
+                v = lane == 0 ? e : v;
             } else if (EX[i] == -1) {
+                const S e = lm[wnext][xdir_slot(i)];
                 v = wave_shift1<false>(v);
-                if (lane == 63) v = lm[wnext][xdir_slot(i)];
+                v = lane == 63 ? e : v;
             }
             f[i] = P::dec(v, OPP[i]);
         }
@@ -747,10 +755,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
         decompose(g, c2, x2, y2, z2);
         const int Ys[3] = {(y2 == 0 ? g.ny - 1 : y2 - 1) * g.nx, y2 * g.nx, (y2 == g.ny - 1 ? 0 : y2 + 1) * g.nx};
         const int Zs[3] = {(z2 == 0 ? g.nz - 1 : z2 - 1) * pl, z2 * pl, (z2 == g.nz - 1 ? 0 : z2 + 1) * pl};
+        if (lane == 63) {
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-            if (lane == 63) xp[w][k] = fo[XPLUS[k]];
-            if (lane == 0) xm[w][k] = fo[XMINUS[k]];
+            for (int k = 0; k < 5; k++) xp[w][k] = fo[XPLUS[k]];
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) xm[w][k] = fo[XMINUS[k]];
         }
         __syncthreads();
 #pragma unroll
@@ -758,11 +769,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<FMT> &a, const int cell
             const uint32_t dst = (uint32_t)(x2 + Ys[1 + EY[i]] + Zs[1 + EZ[i]]);
             T v = fo[i];
             if (EX[i] == 1) {
+                const T e = xp[wprev][xdir_slot(i)];
                 v = wave_shift1<true>(v);
-                if (lane == 0) v = xp[wprev][xdir_slot(i)];
+                v = lane == 0 ? e : v;
             } else if (EX[i] == -1) {
+                const T e = xm[wnext][xdir_slot(i)];
                 v = wave_shift1<false>(v);
-                if (lane == 63) v = xm[wnext][xdir_slot(i)];
+                v = lane == 63 ? e : v;
             }
             ios.st(i, dst, P::enc(v, i));
         }
