@@ -855,19 +855,40 @@ __global__ __launch_bounds__(BS) void k_step_batch(BatchArgs<FMT> b) {
     step_cell<FMT, OP, PHASE, ROW, NT>(a, cell);
 }
 
+// One prepass work item: the list slot k of the cell (-1 for padding), the
+// cell (local to its tunnel), the Bouzidi q of direction opp(i) at it, and
+// its tunnel (0 for one lattice).  One 16-byte load gives the lane all it
+// needs before the population loads.
+struct alignas(16) PrepassItem {
+    int32_t k, cell;
+    float qo;
+    int32_t tunnel;
+};
+static_assert(sizeof(PrepassItem) == 16, "PrepassItem is one 16-byte load");
+
+// Work item idx as one 16-byte load.  A buffer-load intrinsic: a plain struct
+// load is split into its fields, and the loads of cell and q are then sunk
+// below the k < 0 test, which costs a second dependent memory round trip.
+// The list holds at most 2^28 items, so the byte offset fits in 32 bits.
+__device__ __forceinline__ PrepassItem load_item(const PrepassItem *items, int idx) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u v = __builtin_amdgcn_raw_buffer_load_b128(
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<PrepassItem *>(items), 0, -1, 0x00020000), (uint32_t)idx * 16u,
+        0, 0);
+    return PrepassItem{(int32_t)v.x, (int32_t)v.y, __uint_as_float(v.z), (int32_t)v.w};
+}
+
 template <int FMT>
 struct BoundArgs {
     const typename Pop<FMT>::S *pop;
-    const int32_t *ovr_cells;
-    const float *ovr_q;
     typename Pop<FMT>::T *ovr_vals;
     int n_ovr;
     int periodic;
     Geo g;
-    // Work list: entries seg[i] .. seg[i+1] - 1 of pairs are the list slots k
-    // whose direction i needs the rule (ovr_bits), each direction's segment
-    // padded with -1 to a multiple of 64.
-    const int32_t *pairs;
+    // Work list: entries seg[i] .. seg[i+1] - 1 of items are the (cell,
+    // direction i) pairs that need the rule (ovr_bits), each direction's
+    // segment padded with k = -1 to a multiple of 64.
+    const PrepassItem *items;
     int seg[20];
 };
 
@@ -915,29 +936,35 @@ static uint32_t special_dirs(int nx, int ny, int nz, int periodic, int cell, con
     return bits;
 }
 
-// The prepass work list of n listed cells with direction sets bits[k]: seg
-// and a malloc'd pairs array (NULL with seg[19] = 0 when empty or on
-// allocation failure; returns 0 on failure).
-static int build_pairs(int n, const uint32_t *bits, int32_t **pairs, int seg[20]) {
-    int total = 0;
+// The prepass work list of n listed cells (cells[k], q planes q[i*qstride +
+// k], tunnel tun[k] or 0 when tun is NULL) with direction sets bits[k]: seg
+// and a malloc'd item array (NULL with seg[19] = 0 when empty).  Returns 1,
+// 0 on allocation failure, or -1 when the list would exceed 2^28 items.
+static int build_items(int n, const uint32_t *bits, const int32_t *cells, const float *q, long long qstride,
+                       const int32_t *tun, PrepassItem **items, int seg[20]) {
+    long long total = 0;
+    *items = NULL;
     for (int i = 0; i < 19; i++) {
-        int c = 0;
+        long long c = 0;
         for (int k = 0; k < n; k++) c += (bits[k] >> i) & 1;
-        seg[i] = total;
-        total += (c + 63) & ~63;
+        seg[i] = (int)total;
+        total += (c + 63) & ~63LL;
+        if (total > (1LL << 28)) {
+            seg[19] = 0;
+            return -1;
+        }
     }
-    seg[19] = total;
-    *pairs = NULL;
+    seg[19] = (int)total;
     if (total == 0) return 1;
-    int32_t *p = (int32_t *)malloc(sizeof(int32_t) * (size_t)total);
+    PrepassItem *p = (PrepassItem *)malloc(sizeof(PrepassItem) * (size_t)total);
     if (!p) return 0;
     for (int i = 0; i < 19; i++) {
         int w = seg[i];
         for (int k = 0; k < n; k++)
-            if ((bits[k] >> i) & 1) p[w++] = k;
-        while (w < seg[i + 1]) p[w++] = -1;
+            if ((bits[k] >> i) & 1) p[w++] = PrepassItem{k, cells[k], q[OPP[i] * qstride + k], tun ? tun[k] : 0};
+        while (w < seg[i + 1]) p[w++] = PrepassItem{-1, 0, 0.0f, 0};
     }
-    *pairs = p;
+    *items = p;
     return 1;
 }
 
@@ -957,29 +984,26 @@ __global__ __launch_bounds__(BLOCK) void k_prepass(BoundArgs<FMT> a) {
     const int idx = blockIdx.x * BLOCK + threadIdx.x;
     const int i = prepass_dir(a, idx);
     if (i >= 19) return;
-    const int k = a.pairs[idx];
-    if (k < 0) return;
+    const PrepassItem it = load_item(a.items, idx);
+    if (it.k < 0) return;
     int x, y, z;
-    decompose(a.g, a.ovr_cells[k], x, y, z);
-    const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
-    a.ovr_vals[(long long)i * a.n_ovr + k] = prepass_value<FMT, STATE>(a.pop, a.g, a.periodic, x, y, z, i, qo);
+    decompose(a.g, it.cell, x, y, z);
+    a.ovr_vals[(long long)i * a.n_ovr + it.k] = prepass_value<FMT, STATE>(a.pop, a.g, a.periodic, x, y, z, i, it.qo);
 }
 
-// Batched boundary prepass: list entry k belongs to tunnel ovr_tunnel[k].
+// Batched boundary prepass: each work item names its tunnel.
 template <int FMT, int STATE>
-__global__ __launch_bounds__(BLOCK) void k_prepass_batch(BoundArgs<FMT> a, const int32_t *ovr_tunnel,
-                                                         const TunnelDesc *desc) {
+__global__ __launch_bounds__(BLOCK) void k_prepass_batch(BoundArgs<FMT> a, const TunnelDesc *desc) {
     const int idx = blockIdx.x * BLOCK + threadIdx.x;
     const int i = prepass_dir(a, idx);
     if (i >= 19) return;
-    const int k = a.pairs[idx];
-    if (k < 0) return;
-    const TunnelDesc d = desc[ovr_tunnel[k]];
+    const PrepassItem it = load_item(a.items, idx);
+    if (it.k < 0) return;
+    const TunnelDesc d = desc[it.tunnel];
     int x, y, z;
-    decompose(d.g, a.ovr_cells[k], x, y, z);
-    const float qo = a.ovr_q[(long long)OPP[i] * a.n_ovr + k];
-    a.ovr_vals[(long long)i * a.n_ovr + k] =
-        prepass_value<FMT, STATE>(a.pop + d.popOff, d.g, a.periodic, x, y, z, i, qo);
+    decompose(d.g, it.cell, x, y, z);
+    a.ovr_vals[(long long)i * a.n_ovr + it.k] =
+        prepass_value<FMT, STATE>(a.pop + d.popOff, d.g, a.periodic, x, y, z, i, it.qo);
 }
 
 // ---------------------------------------------------------------- force
@@ -1679,8 +1703,8 @@ struct LbmDev {
     float *ovr_q;
     void *ovr_vals;
     uint32_t *ovr_bits;  // per listed cell: directions taken from ovr_vals
-    int32_t *ovr_pairs;  // prepass work list (see BoundArgs)
-    int pair_cap;
+    PrepassItem *ovr_items;  // prepass work list (see BoundArgs)
+    int item_cap;
     int seg[20];
     int state;  // -1 uninitialised, ST_INIT / ST_A / ST_B
     hipStream_t stream;
@@ -1824,7 +1848,7 @@ extern "C" void lbmdev_destroy(LbmDev *d) {
     hipFree(d->ovr_q);
     hipFree(d->ovr_vals);
     hipFree(d->ovr_bits);
-    hipFree(d->ovr_pairs);
+    hipFree(d->ovr_items);
     hipFree(d->pd);
     hipFree(d->pi);
     hipFree(d->resd);
@@ -1844,15 +1868,17 @@ extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, c
     uint8_t *mask = (uint8_t *)malloc((size_t)nc);
     int32_t *index = (int32_t *)malloc((size_t)nc * 4);
     uint32_t *bits = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n_ovr > 0 ? n_ovr : 1));
-    int32_t *pairs = NULL;
+    PrepassItem *items = NULL;
     int seg[20];
     if (mask && index && bits)
         for (int k = 0; k < n_ovr; k++) bits[k] = special_dirs(d->nx, d->ny, d->nz, periodic, ovr_cells[k], ovr_q, n_ovr, k);
-    if (!mask || !index || !bits || !build_pairs(n_ovr, bits, &pairs, seg)) {
+    const int built = mask && index && bits ? build_items(n_ovr, bits, ovr_cells, ovr_q, n_ovr, NULL, &items, seg) : 0;
+    if (built <= 0) {
         free(mask);
         free(index);
         free(bits);
-        snprintf(g_err, sizeof(g_err), "host allocation failed");
+        snprintf(g_err, sizeof(g_err), built < 0 ? "boundary work list exceeds 2^28 (cell, direction) pairs"
+                                                  : "host allocation failed");
         return 0;
     }
     for (int c = 0; c < nc; c++) {
@@ -1878,7 +1904,7 @@ extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, c
             free(mask);
             free(index);
             free(bits);
-            free(pairs);
+            free(items);
             snprintf(g_err, sizeof(g_err), "device allocation of the boundary list failed");
             return 0;
         }
@@ -1889,26 +1915,26 @@ extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, c
             free(mask);
             free(index);
             free(bits);
-            free(pairs);
+            free(items);
             d->ovr_cap = 0;
             snprintf(g_err, sizeof(g_err), "device allocation of the boundary list failed");
             return 0;
         }
     }
-    if (seg[19] > d->pair_cap) {
-        hipFree(d->ovr_pairs);
-        d->ovr_pairs = NULL;
-        d->pair_cap = 0;
+    if (seg[19] > d->item_cap) {
+        hipFree(d->ovr_items);
+        d->ovr_items = NULL;
+        d->item_cap = 0;
         const int cap = seg[19] + 19 * 64;
-        if (hipMalloc((void **)&d->ovr_pairs, (size_t)cap * 4) != hipSuccess) {
+        if (hipMalloc((void **)&d->ovr_items, (size_t)cap * sizeof(PrepassItem)) != hipSuccess) {
             free(mask);
             free(index);
             free(bits);
-            free(pairs);
+            free(items);
             snprintf(g_err, sizeof(g_err), "device allocation of the boundary work list failed");
             return 0;
         }
-        d->pair_cap = cap;
+        d->item_cap = cap;
     }
     memcpy(d->seg, seg, sizeof(seg));
     d->n_ovr = n_ovr;
@@ -1921,11 +1947,11 @@ extern "C" int lbmdev_set_geometry(LbmDev *d, const int32_t *solid, int n_ovr, c
         e = hipMemcpy(d->ovr_q, ovr_q, (size_t)n_ovr * 19 * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && n_ovr > 0) e = hipMemcpy(d->ovr_bits, bits, (size_t)n_ovr * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && seg[19] > 0)
-        e = hipMemcpy(d->ovr_pairs, pairs, (size_t)seg[19] * 4, hipMemcpyHostToDevice);
+        e = hipMemcpy(d->ovr_items, items, (size_t)seg[19] * sizeof(PrepassItem), hipMemcpyHostToDevice);
     free(mask);
     free(index);
     free(bits);
-    free(pairs);
+    free(items);
     if (e != hipSuccess) return set_err("geometry upload", e);
     return 1;
 }
@@ -2024,13 +2050,11 @@ static void launch_prepass(LbmDev *d, int periodic) {
     if (d->n_ovr == 0) return;
     BoundArgs<FMT> a;
     a.pop = (const typename Pop<FMT>::S *)d->pop;
-    a.ovr_cells = d->ovr_cells;
-    a.ovr_q = d->ovr_q;
     a.ovr_vals = (typename Pop<FMT>::T *)d->ovr_vals;
     a.n_ovr = d->n_ovr;
     a.periodic = periodic;
     a.g = make_geo(d->nx, d->ny, d->nz, d->stride);
-    a.pairs = d->ovr_pairs;
+    a.items = d->ovr_items;
     memcpy(a.seg, d->seg, sizeof(a.seg));
     if (a.seg[19] == 0) return;
     const int nb = (a.seg[19] + BLOCK - 1) / BLOCK;
@@ -2491,11 +2515,11 @@ struct LbmBatchDev {
     float4 *vel;
     int n_ovr;
     int *kstart;  // nt + 1 list offsets per tunnel (host)
-    int32_t *ovr_cells, *ovr_tunnel;
+    int32_t *ovr_cells;
     float *ovr_q;
     void *ovr_vals;
     uint32_t *ovr_bits;  // per listed cell: directions taken from ovr_vals
-    int32_t *ovr_pairs;  // prepass work list (see BoundArgs)
+    PrepassItem *ovr_items;  // prepass work list (see BoundArgs)
     int seg[20];
     int32_t *block_tunnel;
     int nblocks, bs;
@@ -2639,11 +2663,10 @@ extern "C" void lbmbatch_destroy(LbmBatchDev *b) {
     hipFree(b->vel);
     hipFree(b->desc_d);
     hipFree(b->ovr_cells);
-    hipFree(b->ovr_tunnel);
     hipFree(b->ovr_q);
     hipFree(b->ovr_vals);
     hipFree(b->ovr_bits);
-    hipFree(b->ovr_pairs);
+    hipFree(b->ovr_items);
     hipFree(b->block_tunnel);
     hipFree(b->pd);
     hipFree(b->pi);
@@ -2673,19 +2696,23 @@ extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const
     int32_t *index = (int32_t *)malloc((size_t)nc * 4);
     int32_t *tun = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ntot > 0 ? ntot : 1));
     uint32_t *bits = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(ntot > 0 ? ntot : 1));
-    int32_t *pairs = NULL;
+    PrepassItem *items = NULL;
     int seg[20];
-    if (bits)
+    if (bits && tun)
         for (int t = 0; t < b->nt; t++)
-            for (int k = b->kstart[t]; k < b->kstart[t + 1]; k++)
+            for (int k = b->kstart[t]; k < b->kstart[t + 1]; k++) {
                 bits[k] = special_dirs(b->desc_h[t].g.nx, b->desc_h[t].g.ny, b->desc_h[t].g.nz, periodic, ovr_cells[k],
                                        ovr_q, ntot, k);
-    if (!mask || !index || !tun || !bits || !build_pairs(ntot, bits, &pairs, seg)) {
+                tun[k] = t;
+            }
+    const int built = mask && index && tun && bits ? build_items(ntot, bits, ovr_cells, ovr_q, ntot, tun, &items, seg) : 0;
+    if (built <= 0) {
         free(mask);
         free(index);
         free(tun);
         free(bits);
-        snprintf(g_err, sizeof(g_err), "host allocation failed");
+        snprintf(g_err, sizeof(g_err), built < 0 ? "boundary work list exceeds 2^28 (cell, direction) pairs"
+                                                  : "host allocation failed");
         return 0;
     }
     for (long long c = 0; c < nc; c++) {
@@ -2697,41 +2724,37 @@ extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const
             const long long c = b->desc_h[t].cellOff + (long long)ovr_cells[k];
             mask[c] |= LBMDEV_MASK_OVR;
             index[c] = k;
-            tun[k] = t;
         }
     hipFree(b->ovr_cells);
-    hipFree(b->ovr_tunnel);
     hipFree(b->ovr_q);
     hipFree(b->ovr_vals);
     hipFree(b->ovr_bits);
-    hipFree(b->ovr_pairs);
-    b->ovr_cells = b->ovr_tunnel = NULL;
+    hipFree(b->ovr_items);
+    b->ovr_cells = NULL;
     b->ovr_q = NULL;
     b->ovr_vals = NULL;
     b->ovr_bits = NULL;
-    b->ovr_pairs = NULL;
+    b->ovr_items = NULL;
     hipError_t e = hipSuccess;
     const size_t n1 = (size_t)(ntot > 0 ? ntot : 1);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_bits, n1 * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_pairs, (size_t)(seg[19] > 0 ? seg[19] : 1) * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_items, (size_t)(seg[19] > 0 ? seg[19] : 1) * sizeof(PrepassItem));
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_cells, n1 * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_tunnel, n1 * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_q, n1 * 19 * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&b->ovr_vals, n1 * 19 * b->telem);
     if (e == hipSuccess) e = hipMemcpy(b->mask, mask, (size_t)nc, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->ovr_index, index, (size_t)nc * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_cells, ovr_cells, (size_t)ntot * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_tunnel, tun, (size_t)ntot * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && ntot > 0)
         e = hipMemcpy(b->ovr_q, ovr_q, (size_t)ntot * 19 * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && ntot > 0) e = hipMemcpy(b->ovr_bits, bits, (size_t)ntot * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && seg[19] > 0)
-        e = hipMemcpy(b->ovr_pairs, pairs, (size_t)seg[19] * 4, hipMemcpyHostToDevice);
+        e = hipMemcpy(b->ovr_items, items, (size_t)seg[19] * sizeof(PrepassItem), hipMemcpyHostToDevice);
     free(mask);
     free(index);
     free(tun);
     free(bits);
-    free(pairs);
+    free(items);
     if (e != hipSuccess) return set_err("batch geometry upload", e);
     memcpy(b->seg, seg, sizeof(seg));
     b->n_ovr = ntot;
@@ -2857,21 +2880,19 @@ static void batch_step_once(LbmBatchDev *b, const LbmBatchParams *p, int wv) {
         if (b->n_ovr > 0 && b->seg[19] > 0) {
             BoundArgs<FMT> a;
             a.pop = (const typename Pop<FMT>::S *)b->pop;
-            a.ovr_cells = b->ovr_cells;
-            a.ovr_q = b->ovr_q;
             a.ovr_vals = (typename Pop<FMT>::T *)b->ovr_vals;
             a.n_ovr = b->n_ovr;
             a.periodic = p->periodicYZ == 1;
             a.g = make_geo(2, 1, 1, 64);
-            a.pairs = b->ovr_pairs;
+            a.items = b->ovr_items;
             memcpy(a.seg, b->seg, sizeof(a.seg));
             const int nb = (a.seg[19] + BLOCK - 1) / BLOCK;
             if (b->state == ST_A)
                 hipLaunchKernelGGL((k_prepass_batch<FMT, ST_A>), dim3(nb), dim3(BLOCK), 0, b->stream, a,
-                                   (const int32_t *)b->ovr_tunnel, (const TunnelDesc *)b->desc_d);
+                                   (const TunnelDesc *)b->desc_d);
             else
                 hipLaunchKernelGGL((k_prepass_batch<FMT, ST_B>), dim3(nb), dim3(BLOCK), 0, b->stream, a,
-                                   (const int32_t *)b->ovr_tunnel, (const TunnelDesc *)b->desc_d);
+                                   (const TunnelDesc *)b->desc_d);
         }
         phase = b->state == ST_A ? 1 : 0;
         use_ovr = 1;
