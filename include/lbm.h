@@ -212,6 +212,22 @@ typedef struct {
 void LBM_AdvectParticlesLive(LBMGrid *grid, Particle *particles, int n, float dt, const LBMParticleFlow *flow,
                              unsigned int seed, unsigned int step);
 
+/* Streamlines of the velocity buffer, the application's visualisation mode 9
+ * (inline GL in main.c:1120-1140 dispatching simulation/shaders/
+ * streamline_trace.comp with STREAMLINE_SEEDS 256 and STREAMLINE_LEN 64,
+ * main.c:775-776).  Seed s starts at x = -2.5 on a gridSide x gridSide
+ * lattice of the y-z cross-section (gridSide = ceil(sqrt(numSeeds)),
+ * y, z = -1.5 + 3 (i + 0.5) / gridSide); each of its traceLen vertices is
+ * (x, y, z, 8 |u|) before one classic RK4 step of ds = 8 / traceLen through
+ * the trilinear sampler clamped to [0.5, N - 1.5] with velocities scaled by
+ * 8; a step shorter than 1e-6 ends the line (the remaining vertices repeat
+ * the position with speed 0); positions stay in the box [-4, 4] x [-2, 2]^2.
+ * `streamlines` is device memory (LBM_DeviceAlloc) of numSeeds * traceLen
+ * float4, the GL buffer the reference binds at binding 11.  Returns 1, or 0
+ * with LBM_GetLastError set (no grid or output, numSeeds or traceLen below
+ * 1, a grid dimension below 2). */
+int LBM_TraceStreamlines(LBMGrid *grid, float *streamlines, int numSeeds, int traceLen);
+
 void *LBM_DeviceAlloc(unsigned long long bytes);
 void LBM_DeviceFree(void *p);
 int LBM_CopyToDevice(void *dst, const void *src, unsigned long long bytes);

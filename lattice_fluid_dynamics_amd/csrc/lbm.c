@@ -844,6 +844,18 @@ void LBM_AdvectParticles(LBMGrid *g, Particle *p, int n, float dt, float velScal
         note_error(lbmdev_last_error());
 }
 
+int LBM_TraceStreamlines(LBMGrid *g, float *out, int numSeeds, int traceLen) {
+    if (!g || !out || !live_dev(g)) {
+        note_error("LBM_TraceStreamlines: no grid or no output buffer");
+        return 0;
+    }
+    if (!lbmdev_trace_streamlines(impl_of(g)->dev, out, numSeeds, traceLen)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    return 1;
+}
+
 void LBM_AdvectParticlesLive(LBMGrid *g, Particle *p, int n, float dt, const LBMParticleFlow *f, unsigned int seed,
                              unsigned int step) {
     if (!g || !p || !f || !live_dev(g)) return;

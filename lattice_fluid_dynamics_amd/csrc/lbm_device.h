@@ -96,6 +96,7 @@ int lbmdev_particles_live(LbmDev *d, void *dev_particles, int n, float dt, const
                           uint32_t seed, uint32_t step);
 int lbmdev_particles_advect(LbmDev *d, void *dev_particles, int n, float dt, float velScale,
                             uint32_t seed, uint32_t step);
+int lbmdev_trace_streamlines(LbmDev *d, void *dev_out, int nseeds, int len);
 
 int lbmdev_synchronize(LbmDev *d);
 /* Times `nsteps` steps with HIP events on the grid's stream.  total_ms is the

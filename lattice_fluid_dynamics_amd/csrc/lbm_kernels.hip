@@ -1723,6 +1723,85 @@ struct LbmDev {
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
 
+// ------------------------------------------------------------ streamlines
+// The streamline tracer of the application's visualisation mode 9
+// (simulation/shaders/streamline_trace.comp:19-107, dispatched by
+// main.c:1120-1140): one lane per seed.  Seeds on a gridSide x gridSide
+// lattice of the plane x = -2.5, classic RK4 with ds = 8 / traceLen through
+// the trilinear sampler clamped to [0.5, N - 1.5], velocities scaled by 8;
+// vertex i = (position, 8 |u|) before step i; a step shorter than 1e-6 ends
+// the line (the rest repeat the position with speed 0); positions clamped to
+// the box.  Every float expression is evaluated in the shader's order
+// without contraction, as the oracle's C restatement is.  The clamp keeps
+// every sample index inside the grid for any position, NaN included
+// (fmaxf/fminf return the non-NaN operand), as long as each dimension is at
+// least 2, which the host checks.
+__device__ __forceinline__ V3 sample_streamline(const float4 *__restrict__ vel, int nx, int ny, int nz, V3 p) {
+#pragma clang fp contract(off)
+    float gx = (p.x + 4.0f) / 8.0f * (float)nx;
+    float gy = (p.y + 2.0f) / 4.0f * (float)ny;
+    float gz = (p.z + 2.0f) / 4.0f * (float)nz;
+    gx = fminf(fmaxf(gx, 0.5f), (float)nx - 1.5f);
+    gy = fminf(fmaxf(gy, 0.5f), (float)ny - 1.5f);
+    gz = fminf(fmaxf(gz, 0.5f), (float)nz - 1.5f);
+    const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+    const int x0 = (int)flx, y0 = (int)fly, z0 = (int)flz;
+    const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1), z1 = min(z0 + 1, nz - 1);
+    const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
+    const long long sy = nx, sz = (long long)nx * ny;
+    const float4 v000 = vel[x0 + y0 * sy + z0 * sz], v100 = vel[x1 + y0 * sy + z0 * sz];
+    const float4 v010 = vel[x0 + y1 * sy + z0 * sz], v110 = vel[x1 + y1 * sy + z0 * sz];
+    const float4 v001 = vel[x0 + y0 * sy + z1 * sz], v101 = vel[x1 + y0 * sy + z1 * sz];
+    const float4 v011 = vel[x0 + y1 * sy + z1 * sz], v111 = vel[x1 + y1 * sy + z1 * sz];
+    const V3 v00 = v3mix(v3(v000.x, v000.y, v000.z), v3(v100.x, v100.y, v100.z), fx);
+    const V3 v10 = v3mix(v3(v010.x, v010.y, v010.z), v3(v110.x, v110.y, v110.z), fx);
+    const V3 v01 = v3mix(v3(v001.x, v001.y, v001.z), v3(v101.x, v101.y, v101.z), fx);
+    const V3 v11 = v3mix(v3(v011.x, v011.y, v011.z), v3(v111.x, v111.y, v111.z), fx);
+    return v3mix(v3mix(v00, v10, fy), v3mix(v01, v11, fy), fz);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_streamlines(const float4 *__restrict__ vel, int nx, int ny, int nz,
+                                                       int nseeds, int len, float4 *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= nseeds) return;
+    const int side = (int)ceilf(sqrtf((float)nseeds));
+    const int iy = s / side, iz = s % side;
+    V3 pos = v3(-2.5f, -1.5f + 3.0f * ((float)iy + 0.5f) / (float)side, -1.5f + 3.0f * ((float)iz + 0.5f) / (float)side);
+    const float ds = 8.0f / (float)len, h = 0.5f * ds, c = ds / 6.0f;
+    float4 *o = out + (long long)s * len;
+    for (int i = 0; i < len; i++) {
+        const V3 v = sample_streamline(vel, nx, ny, nz, pos);
+        o[i] = make_float4(pos.x, pos.y, pos.z, v3len(v) * 8.0f);
+        const V3 k1 = v3scale(v, 8.0f);
+        const V3 k2 = v3scale(sample_streamline(vel, nx, ny, nz, v3add(pos, v3scale(k1, h))), 8.0f);
+        const V3 k3 = v3scale(sample_streamline(vel, nx, ny, nz, v3add(pos, v3scale(k2, h))), 8.0f);
+        const V3 k4 = v3scale(sample_streamline(vel, nx, ny, nz, v3add(pos, v3scale(k3, ds))), 8.0f);
+        const V3 st = v3scale(v3add(v3add(v3add(k1, v3scale(k2, 2.0f)), v3scale(k3, 2.0f)), k4), c);
+        if (v3len(st) < 1e-6f) {
+            for (int j = i + 1; j < len; j++) o[j] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+            return;
+        }
+        pos = v3add(pos, st);
+        pos = v3(fminf(fmaxf(pos.x, -4.0f), 4.0f), fminf(fmaxf(pos.y, -2.0f), 2.0f), fminf(fmaxf(pos.z, -2.0f), 2.0f));
+    }
+}
+
+extern "C" int lbmdev_trace_streamlines(LbmDev *d, void *dev_out, int nseeds, int len) {
+    if (d->nx < 2 || d->ny < 2 || d->nz < 2) {
+        snprintf(g_err, sizeof(g_err), "LBM_TraceStreamlines: every grid dimension must be at least 2");
+        return 0;
+    }
+    if (nseeds < 1 || len < 1 || !dev_out) {
+        snprintf(g_err, sizeof(g_err), "LBM_TraceStreamlines: needs numSeeds >= 1, traceLen >= 1 and an output");
+        return 0;
+    }
+    hipLaunchKernelGGL(k_streamlines, dim3((nseeds + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, d->stream,
+                       (const float4 *)d->vel, d->nx, d->ny, d->nz, nseeds, len, (float4 *)dev_out);
+    HIPCHECK(hipGetLastError(), "streamline launch");
+    return 1;
+}
+
 // ------------------------------------------------ staging without allocation
 // A file-scope device array: the staging buffer of lbmdev_init_field_noalloc,
 // which calls no allocator.  Each GPU the library runs on holds its own copy

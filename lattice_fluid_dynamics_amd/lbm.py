@@ -90,6 +90,7 @@ SIGNATURES = {
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
     "LBM_AdvectParticlesLive": (None, [GridP, c_void_p, c_int, c_float, c_void_p, ctypes.c_uint, ctypes.c_uint]),
+    "LBM_TraceStreamlines": (c_int, [GridP, c_void_p, c_int, c_int]),
     "LBM_GetGuardResetCount": (ctypes.c_longlong, [GridP]),
     "LBM_GetLastError": (ctypes.c_char_p, []),
     "LBM_CdMonitorCreate": (c_void_p, [GridP, c_float, c_float, c_int]),
@@ -418,6 +419,27 @@ class DeviceBuffer:
 def LBM_AdvectParticlesLive(g: Grid, parts: "DeviceParticles", dt: float, flow: LBMParticleFlow, seed: int,
                             step: int) -> None:
     load_library().LBM_AdvectParticlesLive(g.ptr, parts.ptr, parts.n, dt, ctypes.byref(flow), seed, step)
+
+
+def LBM_TraceStreamlines(g: Grid, num_seeds: int = 256, trace_len: int = 64) -> np.ndarray:
+    """RK4 streamlines of the grid's velocity buffer (the application's
+    visualisation mode 9): (num_seeds, trace_len, 4) vertices (x, y, z,
+    speed), traced into a device buffer and copied back."""
+    L = load_library()
+    nbytes = 16 * max(num_seeds, 1) * max(trace_len, 1)
+    dev = L.LBM_DeviceAlloc(nbytes)
+    if not dev:
+        raise MemoryError("device allocation for streamlines failed")
+    try:
+        if not L.LBM_TraceStreamlines(g.ptr, dev, num_seeds, trace_len):
+            raise RuntimeError(last_error())
+        L.LBM_Synchronize(g.ptr)
+        out = np.empty((num_seeds, trace_len, 4), np.float32)
+        if not L.LBM_CopyToHost(out.ctypes.data, dev, out.nbytes):
+            raise RuntimeError(last_error())
+        return out
+    finally:
+        L.LBM_DeviceFree(dev)
 
 
 class DeviceParticles:

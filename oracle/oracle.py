@@ -94,6 +94,7 @@ def lib() -> ctypes.CDLL:
             [ctypes.c_float] * 3 + [c_float_p]
         L.or_sample_nearest.argtypes = L.or_sample_trilinear.argtypes
         L.or_particles_seed.argtypes = [c_float_p, ctypes.c_int, ctypes.c_uint32]
+        L.or_trace_streamlines.argtypes = [c_float_p] + [ctypes.c_int] * 5 + [c_float_p]
         L.or_particles_advect.argtypes = [c_float_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_float_p,
                                           ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
                                           ctypes.c_uint32]
@@ -275,6 +276,16 @@ def sample_nearest(vel: np.ndarray, nx, ny, nz, p):
     out = np.zeros(3, np.float32)
     lib().or_sample_nearest(_ptr(v, ctypes.c_float), nx, ny, nz, float(p[0]), float(p[1]), float(p[2]),
                             _ptr(out, ctypes.c_float))
+    return out
+
+
+def trace_streamlines(vel: np.ndarray, nx, ny, nz, num_seeds: int, trace_len: int) -> np.ndarray:
+    """RK4 streamlines of simulation/shaders/streamline_trace.comp through the
+    (ux, uy, uz, rho) field: (num_seeds, trace_len, 4) vertices (x, y, z, speed)."""
+    v = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+    assert v.size == 4 * nx * ny * nz and min(nx, ny, nz) >= 2
+    out = np.zeros((num_seeds, trace_len, 4), np.float32)
+    lib().or_trace_streamlines(_ptr(v, ctypes.c_float), nx, ny, nz, num_seeds, trace_len, _ptr(out, ctypes.c_float))
     return out
 
 

@@ -385,3 +385,85 @@ void or_particles_live(const float *vel, const int *solid, int nx, int ny, int n
         q->life = life;
     }
 }
+
+/*
+ * Streamline tracer: simulation/shaders/streamline_trace.comp:19-107, the
+ * application's visualisation mode 9 (dispatched by main.c:1120-1140 with 256
+ * seeds of 64 vertices).  Seeds sit on a gridSide x gridSide lattice
+ * (gridSide = ceil(sqrt(numSeeds))) of the plane x = -2.5 with
+ * y, z = -1.5 + 3 (i + 0.5) / gridSide (:71-78); classic RK4 with step
+ * ds = 8 / traceLen (:81-95) through the trilinear sampler clamped to
+ * [0.5, N - 1.5] (:29-59), velocities scaled by 8; vertex i is
+ * (position, 8 |u(position)|) before step i (:84-87); a step shorter than
+ * 1e-6 ends the line and the remaining vertices repeat the position with
+ * speed 0 (:97-102); positions are clamped to [-4, 4] x [-2, 2] x [-2, 2]
+ * (:104-105).  out: numSeeds * traceLen vertices of 4 floats.
+ */
+static void sl_sample(const float *vel, int nx, int ny, int nz, const float p[3], float out[3]) {
+    float gx = (p[0] + 4.0f) / 8.0f * (float)nx;
+    float gy = (p[1] + 2.0f) / 4.0f * (float)ny;
+    float gz = (p[2] + 2.0f) / 4.0f * (float)nz;
+    gx = fminf(fmaxf(gx, 0.5f), (float)nx - 1.5f);
+    gy = fminf(fmaxf(gy, 0.5f), (float)ny - 1.5f);
+    gz = fminf(fmaxf(gz, 0.5f), (float)nz - 1.5f);
+    int x0 = (int)floorf(gx), y0 = (int)floorf(gy), z0 = (int)floorf(gz);
+    int x1 = x0 + 1 < nx - 1 ? x0 + 1 : nx - 1;
+    int y1 = y0 + 1 < ny - 1 ? y0 + 1 : ny - 1;
+    int z1 = z0 + 1 < nz - 1 ? z0 + 1 : nz - 1;
+    float fx = gx - floorf(gx), fy = gy - floorf(gy), fz = gz - floorf(gz);
+#define V(a, b, c, k) vel[4 * ((long)(a) + (long)nx * ((b) + (long)ny * (c))) + (k)]
+    for (int k = 0; k < 3; k++) {
+        float v00 = mixf(V(x0, y0, z0, k), V(x1, y0, z0, k), fx);
+        float v10 = mixf(V(x0, y1, z0, k), V(x1, y1, z0, k), fx);
+        float v01 = mixf(V(x0, y0, z1, k), V(x1, y0, z1, k), fx);
+        float v11 = mixf(V(x0, y1, z1, k), V(x1, y1, z1, k), fx);
+        float v0 = mixf(v00, v10, fy);
+        float v1 = mixf(v01, v11, fy);
+        out[k] = mixf(v0, v1, fz);
+    }
+#undef V
+}
+
+void or_trace_streamlines(const float *vel, int nx, int ny, int nz, int numSeeds, int traceLen, float *out) {
+    const int gridSide = (int)ceilf(sqrtf((float)numSeeds));
+    const float ds = 8.0f / (float)traceLen;
+    const float lo[3] = {-4.0f, -2.0f, -2.0f}, hi[3] = {4.0f, 2.0f, 2.0f};
+    for (int s = 0; s < numSeeds; s++) {
+        const int iy = s / gridSide, iz = s % gridSide;
+        float pos[3] = {-2.5f, -1.5f + 3.0f * ((float)iy + 0.5f) / (float)gridSide,
+                        -1.5f + 3.0f * ((float)iz + 0.5f) / (float)gridSide};
+        float *o = out + 4 * (long)s * traceLen;
+        for (int i = 0; i < traceLen; i++) {
+            float v[3], k1[3], k2[3], k3[3], k4[3], q[3], st[3];
+            sl_sample(vel, nx, ny, nz, pos, v);
+            const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * 8.0f;
+            o[4 * i + 0] = pos[0];
+            o[4 * i + 1] = pos[1];
+            o[4 * i + 2] = pos[2];
+            o[4 * i + 3] = speed;
+            const float h = 0.5f * ds;
+            for (int k = 0; k < 3; k++) k1[k] = v[k] * 8.0f;
+            for (int k = 0; k < 3; k++) q[k] = pos[k] + h * k1[k];
+            sl_sample(vel, nx, ny, nz, q, k2);
+            for (int k = 0; k < 3; k++) k2[k] = k2[k] * 8.0f;
+            for (int k = 0; k < 3; k++) q[k] = pos[k] + h * k2[k];
+            sl_sample(vel, nx, ny, nz, q, k3);
+            for (int k = 0; k < 3; k++) k3[k] = k3[k] * 8.0f;
+            for (int k = 0; k < 3; k++) q[k] = pos[k] + ds * k3[k];
+            sl_sample(vel, nx, ny, nz, q, k4);
+            for (int k = 0; k < 3; k++) k4[k] = k4[k] * 8.0f;
+            const float c = ds / 6.0f;
+            for (int k = 0; k < 3; k++) st[k] = c * (k1[k] + 2.0f * k2[k] + 2.0f * k3[k] + k4[k]);
+            if (sqrtf(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]) < 1e-6f) {
+                for (int j = i + 1; j < traceLen; j++) {
+                    o[4 * j + 0] = pos[0];
+                    o[4 * j + 1] = pos[1];
+                    o[4 * j + 2] = pos[2];
+                    o[4 * j + 3] = 0.0f;
+                }
+                break;
+            }
+            for (int k = 0; k < 3; k++) pos[k] = fminf(fmaxf(pos[k] + st[k], lo[k]), hi[k]);
+        }
+    }
+}

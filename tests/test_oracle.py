@@ -600,6 +600,89 @@ def test_rk2_is_second_order_on_a_rotation():
     assert 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5
 
 
+# ------------------------------------------------ streamlines (streamline_trace.comp)
+def _seed_yz(num_seeds):
+    side = int(np.ceil(np.sqrt(num_seeds)))
+    s = np.arange(num_seeds)
+    return -1.5 + 3.0 * (s // side + 0.5) / side, -1.5 + 3.0 * (s % side + 0.5) / side
+
+
+def test_streamline_seeds_and_stagnation():
+    """A resting field: every line stops at its seed (streamline_trace.comp:97-102),
+    and the seeds lie on the ceil(sqrt(n))-sided lattice of x = -2.5 (:71-78)."""
+    nx, ny, nz = 16, 8, 8
+    for num_seeds in (200, 256, 1):
+        out = O.trace_streamlines(np.zeros((nx * ny * nz, 4), np.float32), nx, ny, nz, num_seeds, 7)
+        y, z = _seed_yz(num_seeds)
+        assert np.allclose(out[:, 0, 1], y, atol=1e-6) and np.allclose(out[:, 0, 2], z, atol=1e-6)
+        assert np.all(out[:, :, 0] == -2.5) and np.all(out[:, :, 3] == 0.0)
+        assert np.array_equal(out, np.repeat(out[:, :1], 7, axis=1))
+
+
+def test_streamline_uniform_flow_and_box_clamp():
+    """u = (0.25, 0, 0): every RK4 stage samples 0.25 exactly, so the speed is
+    8 |u| = 2 exactly and x advances ds * 2 = 0.25 per vertex (ds = 8/64)
+    until the box clamps it at x = 4 (:104-105); y and z never move."""
+    nx, ny, nz = 32, 16, 16
+    vel = np.zeros((nx * ny * nz, 4), np.float32)
+    vel[:, 0] = 0.25
+    out = O.trace_streamlines(vel, nx, ny, nz, 64, 64)
+    i = np.arange(64)
+    assert np.all(out[:, :, 3] == 2.0)
+    assert np.allclose(out[:, :, 0], np.minimum(-2.5 + 0.25 * i, 4.0)[None, :], atol=2e-5)
+    assert np.all(out[:, -1, 0] == 4.0)
+    assert np.array_equal(out[:, :, 1], np.repeat(out[:, :1, 1], 64, axis=1))
+
+
+def test_streamline_sampler_clamps_to_half_cells():
+    """streamline_trace.comp:31 clamps the grid position to [0.5, N - 1.5]
+    (not [0, N - 1] like particle_lbm.comp): with ny = 2 every sample sits at
+    gy = 0.5, so u_y = 0.01 j interpolates to 0.005 wherever the seed is."""
+    nx, ny, nz = 16, 2, 8
+    vel = np.zeros((nz, ny, nx, 4), np.float32)
+    vel[:, 1, :, 1] = 0.01
+    out = O.trace_streamlines(vel.reshape(-1, 4), nx, ny, nz, 49, 3)
+    assert np.allclose(out[:, 0, 3], 8 * 0.005, rtol=1e-6)
+    assert np.allclose(out[:, 1, 1] - out[:, 0, 1], np.minimum(8 / 3 * 0.04, 2.0 - out[:, 0, 1]), atol=1e-6)
+
+
+def _rotation_field(nx, ny, nz, omega):
+    """World-space rotation about the x axis, (0, -omega z, omega y) after the
+    tracer's x8 scale, linear in the grid index so the trilinear sampler
+    reproduces it inside the grid (vel[i] sits at world i * 4 / N - 2)."""
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    yw, zw = j * 4.0 / ny - 2.0, k * 4.0 / nz - 2.0
+    vel = np.zeros((nz, ny, nx, 4), np.float32)
+    vel[..., 1] = (-omega / 8.0 * zw).transpose(2, 1, 0)
+    vel[..., 2] = (omega / 8.0 * yw).transpose(2, 1, 0)
+    return vel.reshape(-1, 4)
+
+
+def test_streamline_rk4_is_fourth_order_on_a_rotation():
+    """A rigid rotation of rate 1 about x: the line of a seed at radius r is
+    the circle y + i z = (y0 + i z0) exp(i t).  With ds = 8 / n, the last
+    vertex (t = 8 - ds) of the classic RK4 tracer misses the circle by an
+    error that falls ~16x when ds halves (fourth order; RK2 would fall 4x,
+    and wrong stage weights or a stage at the wrong point break the ratio)."""
+    nx, ny, nz = 32, 64, 64
+    vel = _rotation_field(nx, ny, nz, 1.0)
+    y0, z0 = _seed_yz(64)
+    keep = np.hypot(y0, z0) < 1.3  # lines that stay where the sampler is unclamped
+    errs = []
+    for n in (16, 32):
+        out = O.trace_streamlines(vel, nx, ny, nz, 64, n).astype(np.float64)
+        t = 8.0 - 8.0 / n
+        zc = (y0 + 1j * z0) * np.exp(1j * t)
+        e = np.abs(out[keep, -1, 1] + 1j * out[keep, -1, 2] - zc[keep])
+        assert np.all(out[keep, :, 0] == -2.5)
+        # speed = 8 |u| = omega r at the vertex
+        assert np.allclose(out[keep, -1, 3], np.hypot(out[keep, -1, 1], out[keep, -1, 2]), rtol=2e-3)
+        errs.append(float(e.max()))
+    print("RK4 streamline errors", errs)
+    assert errs[0] > 1e-4
+    assert 12 < errs[0] / errs[1] < 20
+
+
 # ------------------------------------------------ live particle update
 def uniform_vel(nx, ny, nz, u, rho=1.0):
     v = np.zeros((nx * ny * nz, 4), np.float32)
