@@ -26,8 +26,8 @@ boundary prepass, the particle kernel and the host synchronisations).
 `roofline` prices the fused step kernel alone at 152 B/cell/step (19 loads +
 19 stores of FP32) against its average launch time from HIP events recorded on
 the grid's stream around every launch of every 10th frame of the timed region,
-counted back from its last frame (each such frame ends with a host wait, so
-the other frames run without events).  `cpu_baseline` times the oracle (the
+counted back from its last frame, and read after the region (no host wait
+inside it; the other frames run without events).  `cpu_baseline` times the oracle (the
 C/OpenMP restatement of the reference shaders) on a bounded sample of the same
 workload on the host cores.
 
@@ -99,11 +99,10 @@ def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
     """nsteps LBM steps in frames of 10 (particles advected after each frame).
     Frames are queued without host synchronisation (LBM_StepN), as the
     reference's main loop steps between samples.  With event_every = k > 0,
-    every k-th frame counted back from the last runs through LBM_ProfileSteps
-    instead, whose launches are bracketed by HIP events on the grid's stream
-    and which waits for its last event (the wait of the last frame is the
-    region's closing synchronisation; only a sample of the frames carries
-    events, so the other waits stay rare).
+    every k-th frame counted back from the last runs through LBM_StepNTimed
+    instead, whose launches are bracketed by HIP events on the grid's stream;
+    nothing waits for them inside the region: LBM_CollectStepTimings reads
+    them after its closing synchronisation.
     Returns (wall_s, main_ms_sum, boundary_ms_sum, timed_steps)."""
     nx = w["dims"][0]
     vel_scale = 8.0 / nx
@@ -118,10 +117,7 @@ def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
         n = min(10, nsteps - done)
         sample = event_every > 0 and (frames - 1 - frame) % event_every == 0
         if sample:
-            _, mn, bd = lbm.LBM_ProfileSteps(g, n, w["u"], 0.0, 0.0, per_launch=True)
-            main += mn
-            bnd += bd
-            timed += n
+            lbm.LBM_StepNTimed(g, n, w["u"], 0.0, 0.0)
         else:
             lbm.LBM_StepN(g, n, w["u"], 0.0, 0.0)
         done += n
@@ -129,7 +125,10 @@ def run_frames(lbm, g, parts, w, nsteps, step0, event_every=0):
         if parts is not None:
             lbm.LBM_AdvectParticles(g, parts, float(n), vel_scale, 0, step0 + frame)
     lbm.LBM_Synchronize(g)
-    return time.perf_counter() - t0, main, bnd, timed
+    wall = time.perf_counter() - t0
+    if event_every > 0:
+        main, bnd, timed = lbm.LBM_CollectStepTimings(g)
+    return wall, main, bnd, timed
 
 
 def cpu_baseline(lbm, g, w, steps):

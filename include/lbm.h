@@ -245,6 +245,17 @@ void LBM_Synchronize(LBMGrid *grid);
 int LBM_ProfileSteps(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inletVelZ,
                      float *total_ms, float *main_ms, float *boundary_ms);
 
+/* No reference counterpart (timing a sample of the steps of a long run without
+ * host waits inside it): LBM_StepN with HIP events recorded on the grid's
+ * stream around every boundary prepass and fused step kernel of the n steps,
+ * not waited for.  LBM_CollectStepTimings waits for the grid's queued work and
+ * returns main_ms and boundary_ms summed over the launches of every timed step
+ * since the last collection, and their number in launches (any output may be
+ * NULL).  At most 65536 timed steps may wait for a collection.  Both return 1,
+ * or 0 with LBM_GetLastError set. */
+int LBM_StepNTimed(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inletVelZ);
+int LBM_CollectStepTimings(LBMGrid *grid, float *main_ms, float *boundary_ms, int *launches);
+
 /* Number of cells in the compacted boundary list used by the prepass. */
 int LBM_GetBoundaryCellCount(LBMGrid *grid);
 

@@ -702,6 +702,33 @@ int LBM_ProfileSteps(LBMGrid *g, int n, float ux, float uy, float uz, float *tot
     return ok;
 }
 
+int LBM_StepNTimed(LBMGrid *g, int n, float ux, float uy, float uz) {
+    if (!g || n <= 0) return 0;
+    Impl *im = impl_of(g);
+    if (!im->flowReady) {
+        note_error("LBM_Step called before LBM_InitializeFlow");
+        return 0;
+    }
+    if (!ensure_geometry(g)) return 0;
+    LbmStepParams p;
+    step_params(g, ux, uy, uz, &p);
+    if (!lbmdev_step_timed(im->dev, &p, n)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    g->stepCount += n;
+    return 1;
+}
+
+int LBM_CollectStepTimings(LBMGrid *g, float *main_ms, float *boundary_ms, int *launches) {
+    if (!g || !live_dev(g)) return 0;
+    if (!lbmdev_collect_timings(impl_of(g)->dev, main_ms, boundary_ms, launches)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    return 1;
+}
+
 float *LBM_GetVelocityBuffer(LBMGrid *g) { return g ? g->velocityBuffer : NULL; }
 
 void LBM_Synchronize(LBMGrid *g) {

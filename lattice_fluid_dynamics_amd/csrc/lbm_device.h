@@ -104,6 +104,11 @@ int lbmdev_synchronize(LbmDev *d);
  * the fused step kernel and the boundary prepass (either may be NULL). */
 int lbmdev_profile_steps(LbmDev *d, const LbmStepParams *p, int nsteps, float *total_ms,
                          float *main_ms, float *pre_ms);
+/* Steps with per-launch events recorded and not waited for; collect_timings
+ * waits for the stream and returns the sums over all timed steps since the
+ * last collection (launches = their number). */
+int lbmdev_step_timed(LbmDev *d, const LbmStepParams *p, int nsteps);
+int lbmdev_collect_timings(LbmDev *d, float *main_ms, float *pre_ms, int *launches);
 
 int lbmdev_set_block_size(LbmDev *d, int bs);
 /* Times `steps` steps per candidate workgroup size (128, 256, 512), restoring

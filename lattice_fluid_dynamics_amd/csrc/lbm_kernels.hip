@@ -1719,6 +1719,12 @@ struct LbmDev {
     int bs;      // workgroup size of the step kernel
     int nt;      // LBMDEV_NT_A / LBMDEV_NT_B: non-temporal population access per phase
     int remap;   // XCD-contiguous workgroup order in the step kernel
+    // Timed steps not yet collected (lbmdev_step_timed): three events per
+    // step (before the prepass, before and after the step kernel), reused
+    // across collections; tpre[s] = 1 when step s had a prepass.
+    hipEvent_t *tev;
+    unsigned char *tpre;
+    int tcap, tused;  // steps the pool holds / steps recorded
 };
 
 extern "C" const char *lbmdev_last_error(void) { return g_err; }
@@ -1934,6 +1940,9 @@ extern "C" void lbmdev_destroy(LbmDev *d) {
     hipFree(d->resi);
     hipFree(d->feq_d);
     hipFree(d->guard);
+    for (int k = 0; k < 3 * d->tcap; k++) hipEventDestroy(d->tev[k]);
+    free(d->tev);
+    free(d->tpre);
     if (d->stream) hipStreamDestroy(d->stream);
     free(d);
 }
@@ -2302,6 +2311,72 @@ extern "C" int lbmdev_profile_steps(LbmDev *d, const LbmStepParams *p, int nstep
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     return ok;
+}
+
+// Steps with per-launch events and no host wait: the events go to the
+// grid's pool (grown on the host while earlier work may still run) and are
+// read by lbmdev_collect_timings after the caller's own synchronisation, so
+// sampling the launches of a long run adds no idle time to it.
+extern "C" int lbmdev_step_timed(LbmDev *d, const LbmStepParams *p, int nsteps) {
+    if (d->state < 0) {
+        snprintf(g_err, sizeof(g_err), "LBM_Step before LBM_InitializeFlow");
+        return 0;
+    }
+    if (nsteps <= 0) return 1;
+    const int need = d->tused + nsteps;
+    if (need > (1 << 16)) {
+        snprintf(g_err, sizeof(g_err), "more than 65536 timed steps without LBM_CollectStepTimings");
+        return 0;
+    }
+    if (need > d->tcap) {
+        int cap = d->tcap ? d->tcap : 64;
+        while (cap < need) cap *= 2;
+        hipEvent_t *ev = (hipEvent_t *)realloc(d->tev, sizeof(hipEvent_t) * 3 * (size_t)cap);
+        if (!ev) {
+            snprintf(g_err, sizeof(g_err), "host allocation failed");
+            return 0;
+        }
+        d->tev = ev;
+        unsigned char *pre = (unsigned char *)realloc(d->tpre, (size_t)cap);
+        if (!pre) {
+            snprintf(g_err, sizeof(g_err), "host allocation failed");
+            return 0;
+        }
+        d->tpre = pre;
+        for (int k = 3 * d->tcap; k < 3 * cap; k++) {
+            // timing-only events: no system-scope fence when recorded
+            if (hipEventCreateWithFlags(&d->tev[k], hipEventDisableSystemFence) != hipSuccess) {
+                for (int j = 3 * d->tcap; j < k; j++) hipEventDestroy(d->tev[j]);
+                snprintf(g_err, sizeof(g_err), "event creation failed");
+                return 0;
+            }
+        }
+        d->tcap = cap;
+    }
+    const int s0 = d->tused;
+    for (int s = 0; s < nsteps; s++) d->tpre[s0 + s] = s > 0 || d->state != ST_INIT;
+    if (!dispatch_steps(d, p, nsteps, d->tev + 3 * s0)) return 0;
+    d->tused = need;
+    return 1;
+}
+
+extern "C" int lbmdev_collect_timings(LbmDev *d, float *main_ms, float *pre_ms, int *launches) {
+    HIPCHECK(hipStreamSynchronize(d->stream), "synchronize");
+    double sm = 0, sp = 0;
+    for (int s = 0; s < d->tused; s++) {
+        float a = 0, b = 0;
+        if (d->tpre[s]) {
+            HIPCHECK(hipEventElapsedTime(&a, d->tev[3 * s + 0], d->tev[3 * s + 1]), "event time");
+            sp += a;
+        }
+        HIPCHECK(hipEventElapsedTime(&b, d->tev[3 * s + 1], d->tev[3 * s + 2]), "event time");
+        sm += b;
+    }
+    if (main_ms) *main_ms = (float)sm;
+    if (pre_ms) *pre_ms = (float)sp;
+    if (launches) *launches = d->tused;
+    d->tused = 0;
+    return 1;
 }
 
 template <int FMT, int STATE>

@@ -88,6 +88,8 @@ SIGNATURES = {
     "LBM_CopyDeviceToDevice": (c_int, [c_void_p, c_void_p, ctypes.c_ulonglong]),
     "LBM_Synchronize": (None, [GridP]),
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
+    "LBM_StepNTimed": (c_int, [GridP, c_int, c_float, c_float, c_float]),
+    "LBM_CollectStepTimings": (c_int, [GridP, c_float_p, c_float_p, ctypes.POINTER(c_int)]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
     "LBM_AdvectParticlesLive": (None, [GridP, c_void_p, c_int, c_float, c_void_p, ctypes.c_uint, ctypes.c_uint]),
     "LBM_TraceStreamlines": (c_int, [GridP, c_void_p, c_int, c_int]),
@@ -349,6 +351,19 @@ def LBM_ProfileSteps(g: Grid, n: int, ux: float, uy: float, uz: float, per_launc
     if not ok:
         raise RuntimeError(last_error())
     return tot.value, mn.value, bd.value
+
+
+def LBM_StepNTimed(g: Grid, n: int, ux: float, uy: float, uz: float) -> None:
+    if not load_library().LBM_StepNTimed(g.ptr, n, ux, uy, uz):
+        raise RuntimeError(last_error())
+
+
+def LBM_CollectStepTimings(g: Grid):
+    """(main_ms, boundary_ms, launches) over the steps timed since the last call."""
+    mn, bd, n = c_float(), c_float(), c_int()
+    if not load_library().LBM_CollectStepTimings(g.ptr, ctypes.byref(mn), ctypes.byref(bd), ctypes.byref(n)):
+        raise RuntimeError(last_error())
+    return mn.value, bd.value, n.value
 
 
 def LBM_GetBoundaryCellCount(g: Grid) -> int:

@@ -320,6 +320,31 @@ def test_stepn_equals_repeated_step(gpu):
     assert a.stepCount == b.stepCount == 7
 
 
+def test_timed_steps_equal_stepn_and_collect_their_launches(gpu):
+    """LBM_StepNTimed advances the grid exactly as LBM_StepN does; the timings
+    of every timed step since the last collection come back at once (the
+    first step after LBM_InitializeFlow has no prepass), and a collection
+    resets them."""
+    a, _ = make_pair(32, 16, 16, 0.05, setup_sphere(0, 0, 0, 0.6), periodicYZ=0, useMRT=1)
+    b, _ = make_pair(32, 16, 16, 0.05, setup_sphere(0, 0, 0, 0.6), periodicYZ=0, useMRT=1)
+    gpu.LBM_StepNTimed(a, 3, 0.05, 0, 0)
+    gpu.LBM_StepN(a, 2, 0.05, 0, 0)
+    gpu.LBM_StepNTimed(a, 4, 0.05, 0, 0)
+    gpu.LBM_StepN(b, 9, 0.05, 0, 0)
+    main, bnd, n = gpu.LBM_CollectStepTimings(a)
+    assert n == 7 and main > 0 and bnd > 0 and bnd < main
+    fa, fna, va = fields(a)
+    fb, fnb, vb = fields(b)
+    assert np.array_equal(fa, fb) and np.array_equal(fna, fnb) and np.array_equal(va, vb)
+    assert a.stepCount == b.stepCount == 9
+    assert gpu.LBM_CollectStepTimings(a) == (0.0, 0.0, 0)
+    # the pool grows past its first size and is reused
+    gpu.LBM_StepNTimed(a, 150, 0.05, 0, 0)
+    assert gpu.LBM_CollectStepTimings(a)[2] == 150
+    gpu.LBM_StepNTimed(a, 5, 0.05, 0, 0)
+    assert gpu.LBM_CollectStepTimings(a)[2] == 5
+
+
 def test_total_mass_fp64(gpu):
     g, og = make_pair(48, 24, 24, 0.05, setup_sphere(0, 0, 0, 0.5), periodicYZ=1, useMRT=1)
     gpu.LBM_StepN(g, 50, 0.05, 0, 0)
