@@ -345,6 +345,20 @@ def test_timed_steps_equal_stepn_and_collect_their_launches(gpu):
     assert gpu.LBM_CollectStepTimings(a)[2] == 5
 
 
+def test_profile_steps_equal_stepn(gpu):
+    """LBM_ProfileSteps (blocking, with the region's total) advances the grid
+    as LBM_StepN does; its per-launch sums fit inside the region."""
+    a, _ = make_pair(32, 16, 16, 0.05, setup_sphere(0, 0, 0, 0.6), periodicYZ=1, useRegularized=1)
+    b, _ = make_pair(32, 16, 16, 0.05, setup_sphere(0, 0, 0, 0.6), periodicYZ=1, useRegularized=1)
+    total, main, bnd = gpu.LBM_ProfileSteps(a, 6, 0.05, 0, 0)
+    gpu.LBM_StepN(b, 6, 0.05, 0, 0)
+    assert 0 < bnd < main <= total * 1.001
+    fa, fna, va = fields(a)
+    fb, fnb, vb = fields(b)
+    assert np.array_equal(fa, fb) and np.array_equal(fna, fnb) and np.array_equal(va, vb)
+    assert a.stepCount == 6
+
+
 def test_total_mass_fp64(gpu):
     g, og = make_pair(48, 24, 24, 0.05, setup_sphere(0, 0, 0, 0.5), periodicYZ=1, useMRT=1)
     gpu.LBM_StepN(g, 50, 0.05, 0, 0)
