@@ -200,7 +200,8 @@ typedef struct {
     int useLBM;        /* 1: sample the grid's velocity buffer; 0: the wind only */
     int vizMode;       /* 8: padding1 follows Cp from the density instead of the speed */
     /* collisionMode 2: device copies of the triangles (12 floats each, as
-     * LBM_SetSolidMesh takes them) and of buildCollisionGrid's arrays
+     * LBM_SetSolidMesh takes them, 16-byte aligned as LBM_DeviceAlloc returns
+     * them) and of buildCollisionGrid's arrays
      * (include/lbm_model.h), its origin, cell size and resolution */
     const float *triangles;
     int numTriangles;

@@ -1588,20 +1588,55 @@ __global__ __launch_bounds__(BLOCK) void k_particles_live(Particle *pp, int n, c
                     for (int x = lx; x <= hx && !hit; x++) {
                         const int ci = x + y * R + z * R * R;
                         const int start = f.cellStart[ci], cnt = f.cellCount[ci];
-                        for (int j = 0; j < cnt && !hit; j++) {
-                            // entries outside the arrays (an inconsistent grid) are skipped
-                            if (start + j < 0 || start + j >= f.nIdx) continue;
-                            const int ti = f.triIdx[start + j];
-                            if (ti < 0 || ti >= f.ntri) continue;
-                            const float *t = f.tris + 12 * (long long)ti;
-                            V3 cp, nrm;
-                            const float dist = point_triangle(pos, v3(t[0], t[1], t[2]), v3(t[4], t[5], t[6]),
-                                                              v3(t[8], t[9], t[10]), cp, nrm);
-                            if (dist < radius) {
-                                pos = v3add(cp, v3scale(nrm, radius + 0.001f));
-                                const float vdn = v3dot(v, nrm);
-                                if (vdn < 0.0f) v = v3sub(v, v3scale(nrm, 1.05f * vdn));
-                                hit = true;
+                        // The cell's triangles in rounds of TB: the indices and
+                        // vertices of a round are fetched together, then
+                        // tested in list order (the first hit wins, as in the
+                        // one-at-a-time loop), so a round waits for memory once.
+                        constexpr int TB = 4;
+                        for (int j0 = 0; j0 < cnt && !hit; j0 += TB) {
+                            int tix[TB];
+#pragma unroll
+                            for (int k = 0; k < TB; k++) {
+                                // entries outside the arrays (an inconsistent grid) are skipped
+                                const int e = start + j0 + k;
+                                tix[k] = j0 + k < cnt && e >= 0 && e < f.nIdx ? f.triIdx[e] : -1;
+                                if (tix[k] >= f.ntri) tix[k] = -1;
+                            }
+                            float4 tv[TB][3];
+#pragma unroll
+                            for (int k = 0; k < TB; k++) {
+                                const float4 *t = (const float4 *)(f.tris + 12 * (long long)(tix[k] < 0 ? 0 : tix[k]));
+                                tv[k][0] = t[0];
+                                tv[k][1] = t[1];
+                                tv[k][2] = t[2];
+                            }
+#pragma unroll
+                            for (int k = 0; k < TB; k++) {
+                                if (hit || tix[k] < 0) continue;
+                                // A triangle whose bounding box lies more than
+                                // radius + 1e-3 from the particle on some axis
+                                // cannot give dist < radius (rounding is far
+                                // below the margin), so skipping it changes
+                                // nothing; it saves the full distance test for
+                                // nearly every candidate.
+                                const float m = radius + 1e-3f;
+                                if (pos.x + m < fminf(fminf(tv[k][0].x, tv[k][1].x), tv[k][2].x) ||
+                                    pos.x - m > fmaxf(fmaxf(tv[k][0].x, tv[k][1].x), tv[k][2].x) ||
+                                    pos.y + m < fminf(fminf(tv[k][0].y, tv[k][1].y), tv[k][2].y) ||
+                                    pos.y - m > fmaxf(fmaxf(tv[k][0].y, tv[k][1].y), tv[k][2].y) ||
+                                    pos.z + m < fminf(fminf(tv[k][0].z, tv[k][1].z), tv[k][2].z) ||
+                                    pos.z - m > fmaxf(fmaxf(tv[k][0].z, tv[k][1].z), tv[k][2].z))
+                                    continue;
+                                V3 cp, nrm;
+                                const float dist = point_triangle(pos, v3(tv[k][0].x, tv[k][0].y, tv[k][0].z),
+                                                                  v3(tv[k][1].x, tv[k][1].y, tv[k][1].z),
+                                                                  v3(tv[k][2].x, tv[k][2].y, tv[k][2].z), cp, nrm);
+                                if (dist < radius) {
+                                    pos = v3add(cp, v3scale(nrm, radius + 0.001f));
+                                    const float vdn = v3dot(v, nrm);
+                                    if (vdn < 0.0f) v = v3sub(v, v3scale(nrm, 1.05f * vdn));
+                                    hit = true;
+                                }
                             }
                         }
                     }
@@ -2593,6 +2628,11 @@ extern "C" int lbmdev_particles_live(LbmDev *d, void *dev_particles, int n, floa
     a.gridRes = fl->gridRes;
     if (a.mode == 2 && a.ntri > 0 && a.gridRes > 0 && (!a.tris || !a.cellStart || !a.cellCount || !a.triIdx)) {
         snprintf(g_err, sizeof(g_err), "collision mode 2 needs the triangles and the collision grid on the device");
+        return 0;
+    }
+    if (a.mode == 2 && a.ntri > 0 && ((uintptr_t)a.tris & 15)) {
+        // the kernel reads each 12-float triangle as three float4
+        snprintf(g_err, sizeof(g_err), "collision mode 2 needs a 16-byte aligned triangle array");
         return 0;
     }
     hipLaunchKernelGGL(k_particles_live, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, d->stream,
