@@ -12,8 +12,8 @@ Smagorinsky (the 512x256x256 roofline configuration of the workload list) is
 measured after it and reported under "secondary".
 
 `time_to_converged_cd` is the metric's second half, measured end to end
-after the throughput runs (bounded to ~15 s): the FP32 solver generates
-training fields for 64 seeded bodies, the warm-start network is trained on
+after the throughput runs (bounded to ~20 s): the FP32 solver generates
+training fields for 128 seeded bodies, the warm-start network is trained on
 them by the project's kernels, and on 8 held-out bodies the seconds until Cd
 settles within 1 % of the FP64 cold start are compared for the learned warm
 start with FP16 storage and the plain FP32 cold start.  By default the
@@ -329,8 +329,11 @@ def time_to_converged_cd(dims):
     from lattice_fluid_dynamics_amd import experiments
 
     full = dims[0] >= 256
+    # half resolution: 128 training tunnels and 1500 AdamW steps (about 16 s in
+    # all); 64 tunnels and 600 steps gave a geometric-mean speed-up near 15x
+    # against 20-22x with these
     kw = dict(data_steps=6000, avg_steps=2000, ref_steps=16000, cold_steps=16000, warm_steps=8000, iters=2000,
-              window=10) if full else {}
+              window=10) if full else dict(n_train=128, iters=1500)
     t0 = time.time()
     r = experiments.learned_gain(dims=dims, log=log, **kw)
     return {"metric": "seconds to Cd within 1 % of the FP64 cold start: learned warm start + FP16 storage vs "
