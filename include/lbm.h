@@ -257,6 +257,22 @@ int LBM_ProfileSteps(LBMGrid *grid, int n, float inletVelX, float inletVelY, flo
 int LBM_StepNTimed(LBMGrid *grid, int n, float inletVelX, float inletVelY, float inletVelZ);
 int LBM_CollectStepTimings(LBMGrid *grid, float *main_ms, float *boundary_ms, int *launches);
 
+/* No reference counterpart (checkpoint and resume of long runs; the reference
+ * keeps no solver state across runs).  LBM_SaveState writes into a caller
+ * buffer of LBM_GetStateSize bytes a 64-byte header (magic "LBMSTAT1",
+ * dimensions, storage format, population layout of the last step,
+ * periodicYZ, stepCount, a hash of the geometry) followed by the stored
+ * populations exactly as the device holds them and the velocity buffer.
+ * LBM_LoadState restores it into a grid of the same dimensions,
+ * populationStorage, periodicYZ and geometry (set by the same LBM_SetSolid* /
+ * LBM_AddGroundPlane calls), replacing LBM_InitializeFlow; stepping then
+ * continues bit for bit as the saved grid would.  Both return 1, or 0 with
+ * LBM_GetLastError set (no flow, a short buffer, a mismatch); the collision
+ * flags and tau are the caller's, as for any step. */
+long long LBM_GetStateSize(LBMGrid *grid);
+int LBM_SaveState(LBMGrid *grid, void *buffer, long long bytes);
+int LBM_LoadState(LBMGrid *grid, const void *buffer, long long bytes);
+
 /* Number of cells in the compacted boundary list used by the prepass. */
 int LBM_GetBoundaryCellCount(LBMGrid *grid);
 

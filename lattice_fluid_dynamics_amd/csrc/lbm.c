@@ -838,6 +838,140 @@ int LBM_CopyQToHost(LBMGrid *g, float *out) {
     return 1;
 }
 
+/* ------------------------------------------------------------ checkpoints */
+
+/* Checkpoint container: this 64-byte header, the stored populations exactly
+ * as the device holds them, the velocity buffer. */
+typedef struct {
+    char magic[8];
+    int32_t nx, ny, nz, storage;
+    int32_t layout, periodic;
+    int64_t stepCount;
+    uint64_t geometryHash;
+    uint64_t populationBytes;
+    uint64_t velocityBytes;
+} StateHeader;
+_Static_assert(sizeof(StateHeader) == 64, "checkpoint header is 64 bytes");
+static const char kStateMagic[8] = {'L', 'B', 'M', 'S', 'T', 'A', 'T', '1'};
+
+static uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+/* FNV-1a of the dimensions, the y/z mode, the solid array and every cell's q
+ * row: a checkpoint only continues on the geometry it was taken from. */
+static uint64_t geometry_hash(LBMGrid *g) {
+    Impl *im = impl_of(g);
+    const int32_t head[4] = {g->sizeX, g->sizeY, g->sizeZ, g->periodicYZ == 1};
+    uint64_t h = fnv1a(1469598103934665603ULL, head, sizeof(head));
+    const long n = (long)g->sizeX * g->sizeY * g->sizeZ;
+    h = fnv1a(h, im->solid, sizeof(int32_t) * (size_t)n);
+    for (long c = 0; c < n; c++) {
+        const int r = im->qRow[c];
+        if (r < 0) continue;
+        const int32_t cc = (int32_t)c;
+        h = fnv1a(h, &cc, sizeof(cc));
+        h = fnv1a(h, im->qRows + 19L * r, sizeof(float) * 19);
+    }
+    return h;
+}
+
+long long LBM_GetStateSize(LBMGrid *g) {
+    if (!g || !live_dev(g)) return 0;
+    return (long long)(sizeof(StateHeader) + lbmdev_population_bytes(impl_of(g)->dev) +
+                       (size_t)g->totalCells * 16);
+}
+
+int LBM_SaveState(LBMGrid *g, void *buf, long long bytes) {
+    if (!g || !buf || !live_dev(g)) {
+        note_error("LBM_SaveState: no grid or no buffer");
+        return 0;
+    }
+    Impl *im = impl_of(g);
+    if (!im->flowReady || !ensure_geometry(g)) {
+        note_error("LBM_SaveState: no flow to save (call LBM_InitializeFlow first)");
+        return 0;
+    }
+    if (bytes < LBM_GetStateSize(g)) {
+        note_error("LBM_SaveState: the buffer is smaller than LBM_GetStateSize");
+        return 0;
+    }
+    StateHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, kStateMagic, sizeof(h.magic));
+    h.nx = g->sizeX;
+    h.ny = g->sizeY;
+    h.nz = g->sizeZ;
+    h.storage = lbmdev_storage(im->dev);
+    h.periodic = g->periodicYZ == 1;
+    h.stepCount = g->stepCount;
+    h.geometryHash = geometry_hash(g);
+    h.populationBytes = lbmdev_population_bytes(im->dev);
+    h.velocityBytes = (uint64_t)g->totalCells * 16;
+    char *p = (char *)buf;
+    int layout = 0;
+    if (!lbmdev_save_state(im->dev, p + sizeof(h), p + sizeof(h) + h.populationBytes, &layout)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    h.layout = layout;
+    memcpy(p, &h, sizeof(h));
+    return 1;
+}
+
+int LBM_LoadState(LBMGrid *g, const void *buf, long long bytes) {
+    if (!g || !buf || !live_dev(g)) {
+        note_error("LBM_LoadState: no grid or no buffer");
+        return 0;
+    }
+    StateHeader h;
+    if (bytes < (long long)sizeof(h)) {
+        note_error("LBM_LoadState: truncated state");
+        return 0;
+    }
+    memcpy(&h, buf, sizeof(h));
+    if (memcmp(h.magic, kStateMagic, sizeof(h.magic)) != 0) {
+        note_error("LBM_LoadState: not a saved LBM state");
+        return 0;
+    }
+    if (h.nx != g->sizeX || h.ny != g->sizeY || h.nz != g->sizeZ) {
+        note_error("LBM_LoadState: the state was saved from a grid of other dimensions");
+        return 0;
+    }
+    if (h.storage != g->populationStorage) {
+        note_error("LBM_LoadState: populationStorage differs from the saved state's");
+        return 0;
+    }
+    if (h.periodic != (g->periodicYZ == 1)) {
+        note_error("LBM_LoadState: periodicYZ differs from the saved state's");
+        return 0;
+    }
+    Impl *im = impl_of(g);
+    if (!prepare_flow(g)) return 0;
+    if (h.populationBytes != lbmdev_population_bytes(im->dev) || h.velocityBytes != (uint64_t)g->totalCells * 16 ||
+        (uint64_t)bytes < sizeof(h) + h.populationBytes + h.velocityBytes) {
+        note_error("LBM_LoadState: truncated state");
+        return 0;
+    }
+    if (h.geometryHash != geometry_hash(g)) {
+        note_error("LBM_LoadState: the grid's geometry (solid cells, q, y/z mode) differs from the saved state's");
+        return 0;
+    }
+    const char *p = (const char *)buf;
+    if (!lbmdev_load_state(im->dev, p + sizeof(h), p + sizeof(h) + h.populationBytes, h.layout)) {
+        note_error(lbmdev_last_error());
+        return 0;
+    }
+    im->flowReady = 1;
+    g->stepCount = h.stepCount;
+    return 1;
+}
+
 int LBM_CopyPopulationsToHost(LBMGrid *g, float *f, float *fNew) {
     if (!g || !impl_of(g)->flowReady || !ensure_geometry(g)) return 0;
     return lbmdev_export_populations(impl_of(g)->dev, g->periodicYZ == 1, f, fNew);

@@ -99,6 +99,12 @@ int lbmdev_particles_advect(LbmDev *d, void *dev_particles, int n, float dt, flo
 int lbmdev_trace_streamlines(LbmDev *d, void *dev_out, int nseeds, int len);
 
 int lbmdev_synchronize(LbmDev *d);
+/* Checkpoint state: the stored populations as the device holds them
+ * (population_bytes) and the velocity buffer (16 B per cell), with the layout
+ * tag of the last step; load restores all three. */
+size_t lbmdev_population_bytes(const LbmDev *d);
+int lbmdev_save_state(LbmDev *d, void *pop, void *vel, int *state);
+int lbmdev_load_state(LbmDev *d, const void *pop, const void *vel, int state);
 /* Times `nsteps` steps with HIP events on the grid's stream.  total_ms is the
  * whole region; main_ms / pre_ms are sums of per-launch event pairs around
  * the fused step kernel and the boundary prepass (either may be NULL). */

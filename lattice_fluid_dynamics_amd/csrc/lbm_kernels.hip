@@ -2542,6 +2542,35 @@ extern "C" int lbmdev_total_mass(LbmDev *d, int periodicYZ, double *mass) {
     return 1;
 }
 
+// Raw state for checkpoints: the stored populations exactly as the device
+// holds them (format, padding and the phase layout) and the velocity buffer.
+extern "C" size_t lbmdev_population_bytes(const LbmDev *d) { return (size_t)19 * d->stride * d->elem; }
+
+extern "C" int lbmdev_save_state(LbmDev *d, void *pop, void *vel, int *state) {
+    if (d->state < 0) {
+        snprintf(g_err, sizeof(g_err), "no flow to save: call LBM_InitializeFlow first");
+        return 0;
+    }
+    HIPCHECK(hipMemcpyAsync(pop, d->pop, lbmdev_population_bytes(d), hipMemcpyDeviceToHost, d->stream), "state readback");
+    HIPCHECK(hipMemcpyAsync(vel, d->vel, (size_t)d->ncell * 16, hipMemcpyDeviceToHost, d->stream), "state readback");
+    HIPCHECK(hipStreamSynchronize(d->stream), "state readback");
+    *state = d->state;
+    return 1;
+}
+
+extern "C" int lbmdev_load_state(LbmDev *d, const void *pop, const void *vel, int state) {
+    if (state != ST_INIT && state != ST_A && state != ST_B) {
+        snprintf(g_err, sizeof(g_err), "saved state has an unknown population layout (%d)", state);
+        return 0;
+    }
+    HIPCHECK(hipStreamSynchronize(d->stream), "synchronize");
+    HIPCHECK(hipMemcpyAsync(d->pop, pop, lbmdev_population_bytes(d), hipMemcpyHostToDevice, d->stream), "state upload");
+    HIPCHECK(hipMemcpyAsync(d->vel, vel, (size_t)d->ncell * 16, hipMemcpyHostToDevice, d->stream), "state upload");
+    HIPCHECK(hipStreamSynchronize(d->stream), "state upload");
+    d->state = state;
+    return 1;
+}
+
 extern "C" float *lbmdev_velocity_ptr(LbmDev *d) { return (float *)d->vel; }
 extern "C" int32_t *lbmdev_solid_ptr(LbmDev *d) { return d->solid; }
 

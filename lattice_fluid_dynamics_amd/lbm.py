@@ -89,6 +89,9 @@ SIGNATURES = {
     "LBM_Synchronize": (None, [GridP]),
     "LBM_ProfileSteps": (c_int, [GridP, c_int, c_float, c_float, c_float, c_float_p, c_float_p, c_float_p]),
     "LBM_StepNTimed": (c_int, [GridP, c_int, c_float, c_float, c_float]),
+    "LBM_GetStateSize": (ctypes.c_longlong, [GridP]),
+    "LBM_SaveState": (c_int, [GridP, c_void_p, ctypes.c_longlong]),
+    "LBM_LoadState": (c_int, [GridP, c_void_p, ctypes.c_longlong]),
     "LBM_CollectStepTimings": (c_int, [GridP, c_float_p, c_float_p, ctypes.POINTER(c_int)]),
     "LBM_GetBoundaryCellCount": (c_int, [GridP]),
     "LBM_AdvectParticlesLive": (None, [GridP, c_void_p, c_int, c_float, c_void_p, ctypes.c_uint, ctypes.c_uint]),
@@ -351,6 +354,23 @@ def LBM_ProfileSteps(g: Grid, n: int, ux: float, uy: float, uz: float, per_launc
     if not ok:
         raise RuntimeError(last_error())
     return tot.value, mn.value, bd.value
+
+
+def LBM_SaveState(g: Grid) -> np.ndarray:
+    """The grid's checkpoint (header, stored populations, velocity) as bytes."""
+    L = load_library()
+    n = int(L.LBM_GetStateSize(g.ptr))
+    if n <= 0:
+        raise RuntimeError(last_error())
+    buf = np.empty(n, np.uint8)
+    if not L.LBM_SaveState(g.ptr, buf.ctypes.data, n):
+        raise RuntimeError(last_error())
+    return buf
+
+
+def LBM_LoadState(g: Grid, buf: np.ndarray) -> int:
+    b = np.ascontiguousarray(buf, np.uint8)
+    return int(load_library().LBM_LoadState(g.ptr, b.ctypes.data, b.size))
 
 
 def LBM_StepNTimed(g: Grid, n: int, ux: float, uy: float, uz: float) -> None:

@@ -58,6 +58,10 @@ def test_null_grid_is_accepted_everywhere():
     assert L.LBM_ComputeLiftCoefficient(None, 0.05, 100.0) == 0.0
     buf = np.zeros(4 * 8, np.float32)
     assert L.LBM_CopyVelocityToHost(None, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    assert L.LBM_GetStateSize(None) == 0
+    assert L.LBM_SaveState(None, buf.ctypes.data, buf.nbytes) == 0
+    assert L.LBM_LoadState(None, buf.ctypes.data, buf.nbytes) == 0
+    assert L.LBM_TraceStreamlines(None, buf.ctypes.data, 1, 1) == 0
 
 
 @pytest.mark.gpu
