@@ -1072,12 +1072,12 @@ __device__ __forceinline__ long long fixed_round(T v) {
 // fluid cell with links, F = sum e_i (f_new_i + f_opp(i)), P = sum e_i (feq_i
 // + feq_opp(i)) with rho, u from the velocity buffer.  Per-cell values are
 // accumulated both in FP64 and as the reference's round(x*1e4) integers.
+// Block `blk` of `nblk` over the list: the lane's share of the sums.
 template <int FMT, int STATE>
-__global__ __launch_bounds__(BLOCK) void k_force(ForceArgs<FMT> a) {
+__device__ __forceinline__ void force_block(const ForceArgs<FMT> &a, int blk, int nblk, double (&acc)[6],
+                                            long long (&fix)[7]) {
     typedef typename Pop<FMT>::T T;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    long long fix[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < a.n_ovr; k += gridDim.x * BLOCK) {
+    for (int k = blk * BLOCK + threadIdx.x; k < a.n_ovr; k += nblk * BLOCK) {
         const int cell = a.ovr_cells[k];
         if (a.mask[cell] & (LBMDEV_MASK_BODY | LBMDEV_MASK_GROUND)) continue;
         float q[19];
@@ -1118,14 +1118,21 @@ __global__ __launch_bounds__(BLOCK) void k_force(ForceArgs<FMT> a) {
         }
         fix[3] += 1;
     }
+}
+
+template <int FMT, int STATE>
+__global__ __launch_bounds__(BLOCK) void k_force(ForceArgs<FMT> a) {
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    long long fix[7] = {0, 0, 0, 0, 0, 0, 0};
+    force_block<FMT, STATE>(a, blockIdx.x, gridDim.x, acc, fix);
     block_reduce_store<6, 7>(acc, fix, a.pd, a.pi);
 }
 
 // Sums per-block partials in a fixed order (deterministic): lane l adds
 // blocks l, l+64, ... then lane 0 adds the 64 lane sums in lane order.
 template <int NV, int NI>
-__global__ __launch_bounds__(64) void k_finalize(const double *pd, const long long *pi, int nblocks,
-                                                 double *outd, long long *outi) {
+__device__ __forceinline__ void finalize_body(const double *pd, const long long *pi, int nblocks, double *outd,
+                                              long long *outi) {
     __shared__ double sd[64][NV > 0 ? NV : 1];
     __shared__ long long si[64][NI > 0 ? NI : 1];
     const int l = threadIdx.x;
@@ -1170,6 +1177,12 @@ __global__ __launch_bounds__(64) void k_finalize(const double *pd, const long lo
         for (int j = 0; j < 64; j++) s += si[j][k];
         outi[k] = s;
     }
+}
+
+template <int NV, int NI>
+__global__ __launch_bounds__(64) void k_finalize(const double *pd, const long long *pi, int nblocks,
+                                                 double *outd, long long *outi) {
+    finalize_body<NV, NI>(pd, pi, nblocks, outd, outi);
 }
 
 // ------------------------------------------------- export / diagnostics
@@ -2756,6 +2769,12 @@ struct LbmBatchDev {
     long long *resi;
     unsigned long long *guard;  // stability-guard resets over all tunnels
     int nparts;
+    // One launch for every tunnel's force: ftun[t] = (list start, list
+    // length, first block, blocks) with the block count the per-tunnel launch
+    // would use, fblk_tunnel[block] = its tunnel; fblocks blocks in all.
+    int4 *ftun;
+    int32_t *fblk_tunnel;
+    int fblocks;
     hipGraphExec_t graph[2];
     int graph_ok;
     LbmBatchParams graph_params;
@@ -2890,6 +2909,8 @@ extern "C" void lbmbatch_destroy(LbmBatchDev *b) {
     hipFree(b->ovr_vals);
     hipFree(b->ovr_bits);
     hipFree(b->ovr_items);
+    hipFree(b->ftun);
+    hipFree(b->fblk_tunnel);
     hipFree(b->block_tunnel);
     hipFree(b->pd);
     hipFree(b->pi);
@@ -2982,6 +3003,38 @@ extern "C" int lbmbatch_set_geometry(LbmBatchDev *b, const int32_t *solid, const
     memcpy(b->seg, seg, sizeof(seg));
     b->n_ovr = ntot;
     batch_free_graphs(b);
+    // the one-launch force tables (batch_force_t): each tunnel gets the blocks
+    // its own launch would, min(ceil(n / BLOCK), FORCE_BLOCKS), at least one
+    hipFree(b->ftun);
+    hipFree(b->fblk_tunnel);
+    b->ftun = NULL;
+    b->fblk_tunnel = NULL;
+    b->fblocks = 0;
+    int4 *ft = (int4 *)malloc(sizeof(int4) * (size_t)b->nt);
+    long long total = 0;
+    if (ft) {
+        for (int t = 0; t < b->nt; t++) {
+            const int n = b->kstart[t + 1] - b->kstart[t];
+            int nb = (n + BLOCK - 1) / BLOCK;
+            if (nb > FORCE_BLOCKS) nb = FORCE_BLOCKS;
+            if (nb < 1) nb = 1;
+            ft[t] = make_int4(b->kstart[t], n, (int)total, nb);
+            total += nb;
+        }
+    }
+    int32_t *bt = ft && total <= b->nparts ? (int32_t *)malloc(sizeof(int32_t) * (size_t)total) : NULL;
+    if (bt) {
+        for (int t = 0; t < b->nt; t++)
+            for (int k = 0; k < ft[t].w; k++) bt[ft[t].z + k] = t;
+        // without these tables the per-tunnel launches are used
+        if (hipMalloc((void **)&b->ftun, sizeof(int4) * (size_t)b->nt) == hipSuccess &&
+            hipMalloc((void **)&b->fblk_tunnel, sizeof(int32_t) * (size_t)total) == hipSuccess &&
+            hipMemcpy(b->ftun, ft, sizeof(int4) * (size_t)b->nt, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->fblk_tunnel, bt, sizeof(int32_t) * (size_t)total, hipMemcpyHostToDevice) == hipSuccess)
+            b->fblocks = (int)total;
+    }
+    free(ft);
+    free(bt);
     return 1;
 }
 
@@ -3232,8 +3285,58 @@ extern "C" int lbmbatch_profile(LbmBatchDev *b, const LbmBatchParams *p, int nst
     return ok;
 }
 
+// Every tunnel's force in one launch: block B works as block B - first of its
+// tunnel's launch of the per-tunnel path (the same lanes, stride and block
+// sums, so the per-tunnel results are bitwise those of that path), and
+// k_finalize_batch reduces each tunnel's block sums in block order.
+template <int FMT, int STATE>
+__global__ __launch_bounds__(BLOCK) void k_force_batch(ForceArgs<FMT> base, const TunnelDesc *desc,
+                                                       const int4 *ftun, const int32_t *fblk_tunnel) {
+    const int t = fblk_tunnel[blockIdx.x];
+    const int4 tt = ftun[t];
+    const TunnelDesc d = desc[t];
+    ForceArgs<FMT> a = base;
+    a.pop = base.pop + d.popOff;
+    a.mask = base.mask + d.cellOff;
+    a.ovr_cells = base.ovr_cells + tt.x;
+    a.ovr_q = base.ovr_q + tt.x;
+    a.vel = base.vel + d.cellOff;
+    a.n_ovr = tt.y;
+    a.g = d.g;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    long long fix[7] = {0, 0, 0, 0, 0, 0, 0};
+    force_block<FMT, STATE>(a, (int)blockIdx.x - tt.z, tt.w, acc, fix);
+    block_reduce_store<6, 7>(acc, fix, base.pd, base.pi);
+}
+
+__global__ __launch_bounds__(64) void k_finalize_batch(const double *pd, const long long *pi, const int4 *ftun,
+                                                       double *outd, long long *outi) {
+    const int t = blockIdx.x;
+    const int4 tt = ftun[t];
+    finalize_body<6, 7>(pd + (long long)tt.z * 6, pi + (long long)tt.z * 7, tt.w, outd + 6 * t, outi + 7 * t);
+}
+
 template <int FMT, int STATE>
 static void batch_force_t(LbmBatchDev *b, int periodic) {
+    if (b->fblocks > 0 && b->fblocks <= b->nparts) {
+        ForceArgs<FMT> a;
+        a.pop = (const typename Pop<FMT>::S *)b->pop;
+        a.mask = b->mask;
+        a.ovr_cells = b->ovr_cells;
+        a.ovr_q = b->ovr_q;
+        a.vel = b->vel;
+        a.n_ovr = 0;
+        a.q_stride = b->n_ovr;
+        a.periodic = periodic;
+        a.g = b->desc_h[0].g;
+        a.pd = b->pd;
+        a.pi = b->pi;
+        hipLaunchKernelGGL((k_force_batch<FMT, STATE>), dim3(b->fblocks), dim3(BLOCK), 0, b->stream, a,
+                           (const TunnelDesc *)b->desc_d, (const int4 *)b->ftun, (const int32_t *)b->fblk_tunnel);
+        hipLaunchKernelGGL(k_finalize_batch, dim3(b->nt), dim3(64), 0, b->stream, b->pd, b->pi,
+                           (const int4 *)b->ftun, b->resd, b->resi);
+        return;
+    }
     for (int t = 0; t < b->nt; t++) {
         const int k0 = b->kstart[t], n = b->kstart[t + 1] - k0;
         const TunnelDesc &d = b->desc_h[t];
