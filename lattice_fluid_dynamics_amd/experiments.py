@@ -15,6 +15,12 @@
       ellipsoid or slanted-rear body 10-20 % of ny, u ~ U[0.02,0.08],
       tau ~ U[0.51,0.8]; MRT + Bouzidi + FP16 storage in one batch (one launch per
       step, graph replay, autotuned workgroup); per-tunnel Cd/Cl.
+
+  python -m lattice_fluid_dynamics_amd.experiments particles
+      The particle and streamline kernels on the 512x256x256 car-proxy flow
+      (MRT + Smagorinsky, FP32, 200 steps): 50,000 particles through the RK2
+      trilinear update and the live update in each collision mode, and the
+      256 x 64 RK4 streamlines; host-timed microseconds per call.
 """
 from __future__ import annotations
 
@@ -357,9 +363,62 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
            "pred_u_rel_l2_median": med("pred_u_rel_l2"), "uniform_u_rel_l2_median": med("uniform_u_rel_l2")})
 
 
+def particles_experiment(n: int = 50000, updates: int = 40):
+    """Host-timed calls (after a synchronisation, up to the next one) of the
+    particle and streamline kernels on a developed 512x256x256 flow."""
+    from . import model as M
+
+    g = lbm.LBM_Create(512, 256, 256, 0.005 / 3)
+    g.tau, g.useMRT, g.useSmagorinsky, g.smagorinskyCs, g.periodicYZ = 0.505, 1, 1, 0.1, 1
+    tris, bbox = geometry.car_proxy(seed=0)
+    lbm.LBM_SetSolidMesh(g, tris, bbox)
+    assert lbm.LBM_InitializeFlow(g, 0.05, 0.0, 0.0) == 1
+    lbm.LBM_StepN(g, 200, 0.05, 0.0, 0.0)
+    lbm.LBM_Synchronize(g)
+    lo, hi = tuple(bbox[:3]), tuple(bbox[3:])
+    center = tuple((a + b) / 2 for a, b in zip(lo, hi))
+    grid = M.collision_grid(tris, bbox)
+    keep = [lbm.DeviceBuffer(tris), lbm.DeviceBuffer(grid["cellStart"].astype(np.int32)),
+            lbm.DeviceBuffer(grid["cellCount"].astype(np.int32)), lbm.DeviceBuffer(grid["triIndices"].astype(np.int32))]
+    dgrid = {"cellStart_dev": keep[1].ptr, "cellCount_dev": keep[2].ptr,
+             "triIndices_dev": (keep[3].ptr, grid["triIndices"].size), "origin": grid["origin"], "cell": grid["cell"]}
+    rng = np.random.default_rng(1)
+    p = np.zeros((n, 8), np.float32)
+    p[:, 0], p[:, 1], p[:, 2] = rng.uniform(-4, 4, n), rng.uniform(-1.3, 1.3, n), rng.uniform(-1.3, 1.3, n)
+    p[:, 7] = 1.0
+    out = {"experiment": "particles", "lattice": "512x256x256", "particles": n, "updates": updates,
+           "triangles": int(tris.shape[0])}
+
+    def timed(fn, calls):
+        fn()
+        lbm.LBM_Synchronize(g)
+        t0 = time.perf_counter()
+        for k in range(calls):
+            fn(k)
+        lbm.LBM_Synchronize(g)
+        return round((time.perf_counter() - t0) / calls * 1e6, 1)
+
+    parts = lbm.DeviceParticles(n)
+    parts.upload(p)
+    out["rk2_trilinear_us"] = timed(lambda k=0: lbm.LBM_AdvectParticles(g, parts, 10.0, 8.0 / 512, 0, k), updates)
+    for mode in range(4):
+        flow = lbm.particle_flow((0.4, 0.0, 0.0), lo, hi, center, collision_mode=mode, use_lbm=1,
+                                 triangles=(keep[0].ptr, tris.shape[0]), grid=dgrid)
+        parts.upload(p)
+        out[f"live_mode{mode}_us"] = timed(lambda k=0: lbm.LBM_AdvectParticlesLive(g, parts, 0.016, flow, 3, k),
+                                           updates)
+    parts.free()
+    L = lbm.load_library()
+    buf = L.LBM_DeviceAlloc(16 * 256 * 64)
+    out["streamlines_256x64_us"] = timed(lambda k=0: L.LBM_TraceStreamlines(g.ptr, buf, 256, 64), 20)
+    lbm.LBM_Synchronize(g)
+    L.LBM_DeviceFree(buf)
+    _emit(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["storage", "sweep", "warmstart", "gain"])
+    ap.add_argument("which", choices=["storage", "sweep", "warmstart", "gain", "particles"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--tunnels", type=int, default=64)
     ap.add_argument("--storage", type=int, default=1)
@@ -384,6 +443,8 @@ def main(argv=None):
         storage_experiment(a.steps or 20000)
     elif a.which == "sweep":
         sweep_experiment(a.tunnels, a.steps or 5000, a.storage, tune_steps=a.tune_steps)
+    elif a.which == "particles":
+        particles_experiment()
     elif a.which == "gain":
         gain_experiment(a.train, a.eval, a.iters, data_steps=a.data_steps or 4000, steps=a.cold_steps or 8000,
                         warm_steps=a.warm_steps or 4000)
