@@ -221,6 +221,21 @@ class WarmStart:
             raise RuntimeError(lbm.last_error())
         return out
 
+    def predict_many(self, masks, us, nus) -> list:
+        """predict for many tunnels, max_batch of them per pass of the network."""
+        P = self.packed_positions
+        xin = DeviceArray(2 * P * 32 * self.max_batch)
+        xout = DeviceArray(4 * P * 32 * self.max_batch)
+        out = []
+        for s in range(0, len(masks), self.max_batch):
+            idx = list(range(s, min(s + self.max_batch, len(masks))))
+            packed = np.stack([pack_input(masks[i], us[i], nus[i]) for i in idx])
+            xin.upload(packed)
+            self.forward(xin, len(idx), xout)
+            o = xout.download(np.float32, (len(idx),) + packed.shape[1:4] + (32,))
+            out += [unpack_output(o[k], us[i]) for k, i in enumerate(idx)]
+        return out
+
     def initialize(self, g: lbm.Grid, u: float):
         """LBM_InitializeFlowFromField with this network's prediction for g."""
         if not lib().LBM_WarmStartInitialize(g.ptr, self.ptr, u):
@@ -329,8 +344,9 @@ def solve_fields(specs: Sequence[TunnelSpec], dims=(256, 128, 128), factor: int 
         b, m, _ = build_batch(part, dims, factor, storage)
         b.initialize()
         if ws is not None:
-            for t, sp in enumerate(part):
-                b.initialize_from_field(t, ws.predict(m[t], sp.u, sp.nu))
+            preds = ws.predict_many(m, [sp.u for sp in part], [sp.nu for sp in part])
+            for t in range(len(part)):
+                b.initialize_from_field(t, preds[t])
         t1 = time.time()
         if log:
             log(f"tunnels {c0}..{c0 + len(part) - 1}: set up in {t1 - t0:.1f}s")
@@ -436,11 +452,10 @@ def cd_histories(specs: Sequence[TunnelSpec], dims, steps: int, storage: int, ws
     init_s = 0.0
     predicted = None
     if ws is not None:
-        predicted = []
         t1 = time.perf_counter()
-        for t, sp in enumerate(specs):
-            predicted.append(ws.predict(masks[t], sp.u, sp.nu))
-            b.initialize_from_field(t, predicted[-1])
+        predicted = ws.predict_many(masks, [sp.u for sp in specs], [sp.nu for sp in specs])
+        for t in range(len(specs)):
+            b.initialize_from_field(t, predicted[t])
         init_s = (time.perf_counter() - t1) / len(specs)
     if log:
         log(f"batch of {len(specs)} tunnels ready in {time.time() - t0:.1f}s")

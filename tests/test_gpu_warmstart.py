@@ -366,6 +366,24 @@ def test_warm_start_initialize_equals_predict_then_field(gpu):
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+def test_predict_many_matches_predict(gpu):
+    """Predictions made max_batch at a time (ragged last pass) agree with the
+    one-tunnel path up to the order of the convolutions' FP32 accumulation,
+    which depends on the batch (a few 1e-6 on fields of order 0.05 and 1,
+    against the BF16 network's own error of order 1e-2)."""
+    ws = W.WarmStart(32, 16, 16, max_batch=4, seed=5)
+    rng = np.random.default_rng(3)
+    masks = [(rng.random((16, 16, 32)) < 0.1).astype(np.float32) for _ in range(6)]
+    us = [0.03, 0.04, 0.05, 0.06, 0.07, 0.05]
+    nus = [0.03, 0.05, 0.06, 0.07, 0.08, 0.1]
+    many = ws.predict_many(masks, us, nus)
+    assert len(many) == 6
+    for k in range(6):
+        one = ws.predict(masks[k], us[k], nus[k])
+        assert many[k].shape == one.shape
+        assert float(np.max(np.abs(many[k] - one))) < 2e-5, (k, float(np.max(np.abs(many[k] - one))))
+
+
 def test_good_initial_field_shortens_convergence(gpu):
     """Started from the converged coarse field of a cold run, a tunnel's
     windowed Cd is within 1 % of the cold run's converged value far sooner."""
