@@ -1,17 +1,17 @@
 """Measured workloads beyond the roofline bench, each printing JSON lines.
 
   python -m lattice_fluid_dynamics_amd.experiments storage [--steps 20000]
-      Config 2: 256x128x128 MRT + Smagorinsky (Cs 0.1), tau 0.52, inlet 0.06,
+      Storage formats: 256x128x128 MRT + Smagorinsky (Cs 0.1), tau 0.52, inlet 0.06,
       Ahmed-like body (64x24x18 cells, front radius 6, 25 deg slant, 2-cell
       clearance, ~20k triangles, Bouzidi).  Cold starts with FP64, FP32, FP16
       and BF16 population storage; Cd/Cl with the pressure/friction split, and
       steps/seconds until the windowed Cd settles within 1 % of the FP64 value.
 
   python -m lattice_fluid_dynamics_amd.experiments warmstart [--train 384 --eval 32 --iters 3000]
-      Config 5: learned warm start (see warmstart_experiment).
+      The learned warm start (see warmstart_experiment).
 
   python -m lattice_fluid_dynamics_amd.experiments sweep [--tunnels 64 --steps 5000]
-      Config 4: tunnels with nx ~ U[64,192], ny = nz = nx/2, a seeded random box,
+      Batched sweep: 64 tunnels with nx ~ U[64,192], ny = nz = nx/2, a seeded random box,
       ellipsoid or slanted-rear body 10-20 % of ny, u ~ U[0.02,0.08],
       tau ~ U[0.51,0.8]; MRT + Bouzidi + FP16 storage in one batch (one launch per
       step, graph replay, autotuned workgroup); per-tunnel Cd/Cl.
@@ -69,7 +69,7 @@ def storage_experiment(steps: int, interval: int = 200):
         tail = slice(-20, None)
         cd = float(np.mean(h.cd[tail]))
         cp = float(np.mean(h.cd_pressure[tail]))
-        _emit({"experiment": "config2-storage", "storage": names[st], "steps": steps, "triangles": ntri,
+        _emit({"experiment": "storage-256x128x128", "storage": names[st], "steps": steps, "triangles": ntri,
                "area": area, "cd": cd, "cl": float(np.mean(h.cl[tail])), "cd_pressure": cp,
                "cd_friction": max(cd - cp, 0.0), "cd_rel_err_vs_fp64": abs(cd - ref) / ref,
                "steps_to_1pct_of_fp64": t, "seconds_to_1pct": None if t is None else t * h.ms_per_step / 1e3,
@@ -124,7 +124,7 @@ def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 
     ms_graph = b.profile(20, use_graph=True) / 20
     t = b.profile(steps, use_graph=True)
     coef = b.coefficients()
-    _emit({"experiment": "config4-sweep", "tunnels": ntun, "total_cells": cells, "steps": steps,
+    _emit({"experiment": "sweep-summary", "tunnels": ntun, "total_cells": cells, "steps": steps,
            "storage": {0: "fp32", 1: "fp16", 2: "bf16"}[storage], "setup_s": round(setup_s, 1),
            "autotune_block": best, "autotune_streaming_mode": b.kernel_config()[1],
            "autotune_xcd_contiguous": b.kernel_config()[2],
@@ -134,7 +134,7 @@ def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 
     for k, m in enumerate(meta):
         m.update({"cd": float(coef[k, 0]), "cl": float(coef[k, 1]), "cd_pressure": float(coef[k, 2]),
                   "cd_friction": float(coef[k, 3])})
-        _emit({"experiment": "config4-tunnel", **m})
+        _emit({"experiment": "sweep-tunnel", **m})
 
 
 def _steps_to_1pct(h, ref, window):
@@ -246,7 +246,7 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
                          dims=(256, 128, 128), factor: int = 4, phase: str = "all", warm_fp32: bool = True,
                          reference: str = "fp32", eval_offset: int = 0, bootstrap: int = 0, boot_steps: int = 1200,
                          boot_avg: int = 1200, boot_iters: int = 1500):
-    """Config 5: learned warm start at 256x128x128 (coarse lattice 64x32x32).
+    """The learned warm start at 256x128x128 (coarse lattice 64x32x32).
 
     Training tunnels use body seeds 0..n_train-1, held-out tunnels seeds
     100000..; inlet speed U[0.03, 0.07] and tau U[0.6, 0.9] per seed (MRT,
@@ -297,7 +297,7 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
         train_s = time.time() - t0
         if weights_out:
             ws.save(weights_out)
-        _emit({"experiment": "config5-train", "train_tunnels": n_train, "samples_with_mirrors": data.n,
+        _emit({"experiment": "warmstart-train", "train_tunnels": n_train, "samples_with_mirrors": data.n,
                "cold_data_tunnels": n_cold, "bootstrap": {"steps": boot_steps, "avg_steps": boot_avg,
                                                           "iters": boot_iters} if n_cold < n_train else None,
                "lattice": f"{nx}x{ny}x{nz}", "coarse": f"{nx // factor}x{ny // factor}x{nz // factor}",
@@ -342,7 +342,7 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
         row["pred_u_rel_l2"] = round(float(du), 4)
         row["uniform_u_rel_l2"] = round(float(du0), 4)
         rows.append(row)
-        _emit({"experiment": "config5-tunnel", **row})
+        _emit({"experiment": "warmstart-tunnel", **row})
 
     def med(key):
         v = [r[key] for r in rows if r[key] is not None]
@@ -350,7 +350,7 @@ def warmstart_experiment(n_train: int, n_eval: int, iters: int, data_steps: int 
 
     ratios = [r["cold_fp32_s"] / r["warm_fp16_s"] for r in rows
               if r["cold_fp32_s"] is not None and r["warm_fp16_s"] is not None]
-    _emit({"experiment": "config5-summary", "eval_tunnels": n_eval, "eval_seeds_from": 100000 + eval_offset,
+    _emit({"experiment": "warmstart-summary", "eval_tunnels": n_eval, "eval_seeds_from": 100000 + eval_offset,
            "reference": reference, "window_samples": window,
            "sample_interval": 100,
            **{k + "_ms_per_step_per_tunnel": round(r["ms_per_step"] / n_eval, 5) for k, r in runs.items()},

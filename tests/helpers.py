@@ -5,7 +5,7 @@ import numpy as np
 from lattice_fluid_dynamics_amd import lbm
 from oracle import oracle as O
 
-# config 1 box (cells [24,39]^3 at 128x64x64)
+# the 16^3 box of the 128x64x64 BGK case (cells [24,39]^3)
 C1_BOX = (-2.5, -0.5, -0.5, -1.53125, 0.46875, 0.46875)
 
 FLAG_FIELDS = ("useRegularized", "useMRT", "useSmagorinsky", "smagorinskyCs", "periodicYZ")

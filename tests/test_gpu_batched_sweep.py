@@ -7,7 +7,7 @@ from HIP graphs.
 
 After 300 steps every tunnel's populations (both of the reference's
 buffers), velocity field and FP64 force sums equal, bit for bit, those of the
-same tunnel run alone through LBM_StepN -- whose FP16 results the config-2
+same tunnel run alone through LBM_StepN -- whose FP16 results the 128x64x64 storage-format
 and Taylor-Green tests tie to the oracle -- and its Cd/Cl are finite."""
 import numpy as np
 import pytest

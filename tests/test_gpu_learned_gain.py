@@ -1,6 +1,6 @@
 """What the trained warm-start network gains over the method without it.
 
-End to end on the GPU, at 128x64x64 tunnels (config 5 at half resolution, so
+End to end on the GPU, at 128x64x64 tunnels (the learned warm start at half resolution, so
 that data generation, training and evaluation take seconds):
 
 1. the FP32 solver produces the training data: 64 seeded random bodies

@@ -184,7 +184,7 @@ def test_first_steps_match_oracle(gpu, periodic, op):
 
 
 def test_config1_bgk_1000_steps(gpu):
-    """Config 1: 128x64x64 BGK, tau 0.6, u=0.05, periodic y/z, 16^3 box,
+    """128x64x64 BGK, tau 0.6, u=0.05, periodic y/z, 16^3 box,
     1000 steps; fields, Cd and Cl against the FP32 oracle."""
     g, og = make_pair(128, 64, 64, 0.1 / 3, setup_aabb(C1_BOX), tau=0.6, periodicYZ=1)
     A = gpu.LBM_ComputeProjectedArea(g, 0)
@@ -192,7 +192,7 @@ def test_config1_bgk_1000_steps(gpu):
     gpu.LBM_StepN(g, 1000, 0.05, 0, 0)
     og.step(0.05, 0, 0, n=1000)
     d = compare(g, og)
-    print("config1 field differences", d)
+    print("128x64x64 BGK field differences", d)
     assert d["u"] < TOL and d["rho"] < TOL and d["f"] < TOL and d["f_new"] < TOL
     # forces: reference fixed point and FP64 sums
     ex_o, fix_o = og.force()
@@ -227,7 +227,7 @@ def test_config1_bgk_1000_steps(gpu):
     errs = {"u_gpu32": rel_l2(v32[:, :3], v64[:, :3]), "u_oracle32": rel_l2(og.vel[:, :3], v64[:, :3]),
             "f_gpu32": rel_l2(f32, f64), "f_oracle32": rel_l2(og.f, f64),
             "Fx_gpu32": abs(F[0] - F64[0]) / abs(F64[0]), "Fx_oracle32": abs(ex_o[0] - F64[0]) / abs(F64[0])}
-    print("config1 error against FP64", errs)
+    print("128x64x64 BGK error against FP64", errs)
     assert errs["u_gpu32"] < TOL and errs["f_gpu32"] < TOL and errs["Fx_gpu32"] < 1e-3
 
 
