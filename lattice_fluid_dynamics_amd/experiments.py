@@ -38,7 +38,7 @@ def _emit(obj):
     print(json.dumps(obj), flush=True)
 
 
-def config2_grid(storage: int):
+def storage_grid(storage: int):
     nx, ny, nz = 256, 128, 128
     g = lbm.LBM_Create(nx, ny, nz, (0.52 - 0.5) / 3)
     g.tau = 0.52
@@ -59,7 +59,7 @@ def storage_experiment(steps: int, interval: int = 200):
     names = {3: "fp64", 0: "fp32", 1: "fp16", 2: "bf16"}
     hist = {}
     for st in (3, 0, 1, 2):
-        g, area, ntri = config2_grid(st)
+        g, area, ntri = storage_grid(st)
         h = convergence.run_history(g, 0.06, area, steps, interval)
         hist[st] = h
         g.free()
@@ -83,7 +83,7 @@ def sweep_bodies(seed: int, nx: int, ny: int, nz: int):
     return geometry.random_body(seed)
 
 
-def config4_tunnels(ntun: int = 64, rng_seed: int = 12345):
+def sweep_tunnels(ntun: int = 64, rng_seed: int = 12345):
     """The tunnels of the batched sweep: nx uniform in [64, 192], ny = nz =
     nx / 2, inlet u uniform in [0.02, 0.08], tau uniform in [0.51, 0.8], body
     seed = tunnel index.  A list of dicts (seed, dims, u, tau)."""
@@ -97,7 +97,7 @@ def config4_tunnels(ntun: int = 64, rng_seed: int = 12345):
     return out
 
 
-def config4_geometry(t):
+def sweep_geometry(t):
     """A grid holding tunnel t's body (solid mask and Bouzidi links), for
     LBM_BatchAddTunnel or a single run; returns (grid, body kind)."""
     nx, ny, nz = t["dims"]
@@ -111,8 +111,8 @@ def sweep_experiment(ntun: int, steps: int, storage: int = 1, tune_steps: int = 
     b = lbm.Batch(storage=storage, useMRT=1, useSmagorinsky=0, periodicYZ=1)
     meta = []
     t0 = time.time()
-    for t in config4_tunnels(ntun):
-        g, kind = config4_geometry(t)
+    for t in sweep_tunnels(ntun):
+        g, kind = sweep_geometry(t)
         b.add(g, t["tau"], t["u"])
         meta.append({**t, "body": kind})
         g.free()

@@ -19,14 +19,14 @@ pytestmark = pytest.mark.gpu
 STEPS = 300
 
 
-def test_config4_batch_of_64_tunnels_equals_single_grids(gpu):
-    tunnels = experiments.config4_tunnels(64)
+def test_batch_of_64_tunnels_equals_single_grids(gpu):
+    tunnels = experiments.sweep_tunnels(64)
     cells = [t["dims"][0] * t["dims"][1] * t["dims"][2] for t in tunnels]
     assert 40e6 < sum(cells) < 55e6
     b = gpu.Batch(storage=gpu.LBM_STORAGE_FP16, useMRT=1, useSmagorinsky=0, periodicYZ=1)
     kinds = []
     for t in tunnels:
-        g, kind = experiments.config4_geometry(t)
+        g, kind = experiments.sweep_geometry(t)
         assert b.add(g, t["tau"], t["u"]) == len(kinds)
         kinds.append(kind)
         g.free()
@@ -41,7 +41,7 @@ def test_config4_batch_of_64_tunnels_equals_single_grids(gpu):
     assert np.all(np.isfinite(coef)) and np.all(coef[:, 0] > 0)
     assert b.guard_resets == 0
     for k, t in enumerate(tunnels):
-        s, _ = experiments.config4_geometry(t)
+        s, _ = experiments.sweep_geometry(t)
         s.tau = t["tau"]
         s.useMRT = 1
         s.periodicYZ = 1

@@ -109,7 +109,7 @@ def test_grid_beyond_2_30_cells(gpu):
     assert abs(large - small) <= 1e-10 and abs(large - 1.0) < 1e-4
 
 
-def test_config3_runs_its_10000_steps(gpu):
+def test_512x256x256_regularized_runs_10000_steps(gpu):
     """The roofline configuration for its configured 10,000 steps, as the
     reference's main loop drives it: 10 LBM steps per frame, then the
     50,000 particles (seeded on the inlet plane from seed 0) advected by RK2

@@ -183,7 +183,7 @@ def test_first_steps_match_oracle(gpu, periodic, op):
         assert d["f"] < 1e-6 and d["f_new"] < 1e-6 and d["u"] < 1e-5 and d["rho"] < 1e-6
 
 
-def test_config1_bgk_1000_steps(gpu):
+def test_bgk_128x64x64_box_1000_steps(gpu):
     """128x64x64 BGK, tau 0.6, u=0.05, periodic y/z, 16^3 box,
     1000 steps; fields, Cd and Cl against the FP32 oracle."""
     g, og = make_pair(128, 64, 64, 0.1 / 3, setup_aabb(C1_BOX), tau=0.6, periodicYZ=1)

@@ -65,7 +65,7 @@ BARS_300 = {0: {"u": 1e-5, "u_inner": 1e-5, "rho": 1e-6, "cd": 1e-4},
             2: {"u": 1e-2, "u_inner": 8e-3, "rho": 4e-4, "cd": 3.5e-2}}
 
 
-def test_config2_shape_storage_formats_against_the_oracles(gpu):
+def test_storage_formats_128x64x64_against_the_oracles(gpu):
     nx, ny, nz = 128, 64, 64
     tris, bbox = half_ahmed(nx, ny, nz)
     setup = setup_mesh(tris, bbox)
@@ -143,7 +143,7 @@ def test_config2_shape_storage_formats_against_the_oracles(gpu):
 BARS_20K = {0: 1e-3, 1: 1.5e-2, 2: 1.8e-1}
 
 
-def test_config2_full_size_20k_steps(gpu):
+def test_storage_fp16_256x128x128_20k_steps(gpu):
     """256x128x128 with the 64x24x18-cell Ahmed-like body (~20k triangles),
     20,000 steps in every storage format.  The flow sheds vortices (Re ~ 580
     on the body length), so instantaneous fields of different formats part

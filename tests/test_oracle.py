@@ -208,7 +208,7 @@ def brute_links(solid, nx, ny, nz, src_ok, dst_ok):
     return out.reshape(-1, 19)
 
 
-def test_aabb_config1_box_cells_links_and_area():
+def test_aabb_16_cube_box_cells_links_and_area():
     g = O.OracleGrid(128, 64, 64, 0.1 / 3)
     g.set_solid_aabb(*C1_BOX)
     s = g.solid.reshape(64, 64, 128)
