@@ -82,6 +82,48 @@ def test_uniform_inflow_is_a_fixed_point_at_full_size(gpu, regularized):
     assert np.max(np.abs(vel[:, 3] - 1.0)) < 2e-6
 
 
+@pytest.mark.parametrize("storage", [0, 1])
+def test_checkpoint_resumes_bit_for_bit_at_full_size(gpu, storage):
+    """LBM_SaveState / LBM_LoadState at 512x256x256 with MRT + Smagorinsky
+    and the car proxy: 2.55 GB of FP32 populations (byte offsets past 2^31)
+    or 1.27 GB of FP16 ones.  Ten steps after a restore, into the saved grid
+    itself and into a fresh grid of the same geometry, reproduce the saved
+    grid's velocity field and force bit for bit."""
+    tris, bbox = geometry.car_proxy(seed=0)
+
+    def make():
+        g = lbm.LBM_Create(NX, NY, NZ, (TAU - 0.5) / 3)
+        configure(g)
+        g.useRegularized = 0
+        g.useMRT = 1
+        g.populationStorage = storage
+        lbm.LBM_SetSolidMesh(g, tris, bbox)
+        return g
+
+    g = make()
+    assert lbm.LBM_InitializeFlow(g, U, 0, 0) == 1
+    lbm.LBM_StepN(g, 21, U, 0, 0)
+    saved = lbm.LBM_SaveState(g)
+    assert saved.size == 64 + NX * NY * NZ * (19 * (4 if storage == 0 else 2) + 16)
+    lbm.LBM_StepN(g, 10, U, 0, 0)
+    want_v = lbm.LBM_CopyVelocityToHost(g)
+    want_f = lbm.LBM_ComputeDragForceF64(g)
+    assert np.all(np.isfinite(want_v)) and want_f[0] != 0.0
+    assert lbm.LBM_LoadState(g, saved) == 1, lbm.last_error()
+    lbm.LBM_StepN(g, 10, U, 0, 0)
+    assert np.array_equal(lbm.LBM_CopyVelocityToHost(g).view(np.uint32), want_v.view(np.uint32))
+    assert np.array_equal(lbm.LBM_ComputeDragForceF64(g), want_f)
+    lbm.LBM_Free(g)
+    h = make()
+    assert lbm.LBM_LoadState(h, saved) == 1, lbm.last_error()
+    assert h.stepCount == 21
+    lbm.LBM_StepN(h, 10, U, 0, 0)
+    assert np.array_equal(lbm.LBM_CopyVelocityToHost(h).view(np.uint32), want_v.view(np.uint32))
+    assert np.array_equal(lbm.LBM_ComputeDragForceF64(h), want_f)
+    print("storage %d: state %.2f GB, Fx %.6e" % (storage, saved.size / 1e9, want_f[0]))
+    lbm.LBM_Free(h)
+
+
 def test_grid_beyond_2_30_cells(gpu):
     """1024 x 1024 x 1025 cells (2^30 + 2^20).  FP32 populations would need
     slot byte offsets past 2^32 within a plane: LBM_InitializeFlow refuses
