@@ -2,7 +2,8 @@
 simulation/shaders/streamline_trace.comp dispatched by main.c:1120-1140)
 against the oracle's C restatement: bit for bit on a developed flow, on a
 rotation written into the velocity buffer, on a resting field (every line
-stops at its seed) and on ragged seed counts and lengths; errors for what the
+stops at its seed), on ragged seed counts and lengths and on ragged walled
+lattices; errors for what the
 kernel cannot trace."""
 import ctypes
 
@@ -15,11 +16,11 @@ from tests.test_oracle import _rotation_field
 pytestmark = pytest.mark.gpu
 
 
-def _flow(gpu, dims=(64, 32, 32), steps=300):
+def _flow(gpu, dims=(64, 32, 32), steps=300, periodic=1):
     g = gpu.LBM_Create(*dims, 0.02)
     g.useMRT = 1
     g.useSmagorinsky = 1
-    g.periodicYZ = 1
+    g.periodicYZ = periodic
     gpu.LBM_SetSolidSphere(g, -1.5, 0.1, -0.05, 0.45)
     assert gpu.LBM_InitializeFlow(g, 0.06, 0.0, 0.0) == 1
     gpu.LBM_StepN(g, steps, 0.06, 0.0, 0.0)
@@ -46,6 +47,19 @@ def test_streamlines_match_oracle_on_a_developed_flow(gpu, num_seeds, trace_len)
     if trace_len > 1:
         assert np.all(got[:, -1, 0] > got[:, 0, 0])
         assert 0.3 < np.median(got[:, 0, 3]) < 0.7  # 8 |u| with u ~ 0.06
+
+
+@pytest.mark.parametrize("dims", [(50, 36, 20), (33, 2, 70)])
+def test_streamlines_match_oracle_on_ragged_walled_lattices(gpu, dims):
+    """Dimensions that are no multiple of the workgroup or of each other, with
+    clamped (walled) y/z faces, and the thinnest lattice the tracer accepts
+    (ny = 2): still bit for bit."""
+    g = _flow(gpu, dims=dims, steps=200, periodic=0)
+    vel = gpu.LBM_CopyVelocityToHost(g)
+    got = gpu.LBM_TraceStreamlines(g, 45, 27)
+    ref = O.trace_streamlines(vel, *dims, 45, 27)
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
 def test_streamlines_match_oracle_on_a_rotation_and_at_rest(gpu):
