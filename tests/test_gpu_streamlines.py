@@ -3,8 +3,7 @@ simulation/shaders/streamline_trace.comp dispatched by main.c:1120-1140)
 against the oracle's C restatement: bit for bit on a developed flow, on a
 rotation written into the velocity buffer, on a resting field (every line
 stops at its seed), on ragged seed counts and lengths and on ragged walled
-lattices; errors for what the
-kernel cannot trace."""
+lattices; errors for what the kernel cannot trace."""
 import ctypes
 
 import numpy as np
