@@ -22,7 +22,7 @@ def make_geometry(lbm, dims, sphere):
 
 
 @pytest.mark.parametrize("periodic", [1, 0])
-@pytest.mark.parametrize("storage", [0, 1, 3])
+@pytest.mark.parametrize("storage", [0, 1, 2, 3])
 def test_batch_equals_single_grids(gpu, periodic, storage):
     b = gpu.Batch(storage=storage, useMRT=1, useSmagorinsky=1, smagorinskyCs=0.1, periodicYZ=periodic)
     singles = []
@@ -114,11 +114,12 @@ def test_batch_coefficients(gpu):
 def test_batch_row_aligned_equals_per_cell_stores(gpu):
     """Tunnels whose rows fill whole workgroups take the row-aligned phase-B
     stores at workgroup 256 and per-cell stores at 128 (128 does not divide
-    the 256-cell rows); both give the same bits, FP32 and FP16 storage."""
+    the 256-cell rows); both give the same bits, FP32, FP16 and BF16
+    storage."""
     tunnels = [((64, 32, 32), (-1.0, 0.1, 0.0, 0.7), 0.56, 0.05),
                ((128, 16, 16), (-0.5, -0.2, 0.1, 0.5), 0.62, 0.03),
                ((256, 16, 8), (0.3, 0.0, 0.2, 0.9), 0.80, 0.07)]
-    for storage in (0, 1):
+    for storage in (0, 1, 2):
         res = []
         for bs in (256, 128):
             b = gpu.Batch(storage=storage, useRegularized=1, useMRT=0, useSmagorinsky=1, smagorinskyCs=0.1,
